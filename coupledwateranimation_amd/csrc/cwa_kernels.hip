@@ -1,0 +1,625 @@
+// cwa_kernels.hip — CDNA4 (gfx950) kernels of the coupled SPH +
+// heightfield-wave step.
+//
+// Data layout in HBM (one scene):
+//   particles  SoA float4 pos / vel / force + int id, in cell-sorted order;
+//              two copies (A = state, B = this step's reorder target)
+//   work       rho_w (float), pw (float2: pressure, 1/rho), flux (float)
+//   grid       key, trank (int per particle), count (int per cell),
+//              offset (int per cell + 1), tmp / list (int per particle)
+//   wave       three W*H fp32 fields (newest, previous, write)
+//
+// Reference kernels replaced (paths relative to the reference root):
+//   ugrid_particles_cs.glsl (hash / count / insert)  -> k_hash_count,
+//       k_scatter_tmp, k_stable_reorder
+//   prefix_sum_cs.glsl + ParallelScan.cpp            -> k_scan_*
+//   rho_pres_comp.glsl                               -> k_density
+//   force_comp.glsl + integrate_comp.glsl            -> k_force_integrate
+//   wave_comp.glsl (EVOLVE, INIT)                    -> k_wave, k_init_wave
+//   particle -> heightfield flux (no reference)      -> k_column_flux
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "cwa_device.hpp"
+#include "cwa_kernels.hpp"
+
+namespace cwa {
+
+// ---------------------------------------------------------------------------
+// utilities
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of int32 (replaces the Blelloch tree of
+// prefix_sum_cs.glsl:18-43, which needs 2 log2(n) dispatches and a
+// power-of-two n).  Reduce-then-scan over tiles of kScanTile elements:
+//   k_scan_reduce  tile sums
+//   (recursive scan of the tile sums)
+//   k_scan_down    tile-local exclusive scan + tile prefix
+// Block-level scans are wave64 shuffle scans plus one LDS exchange of the
+// four wave totals.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exclusive scan of one value per thread across the block; returns the
+// exclusive prefix and the block total.
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int incl = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const int s = wsum[w];
+        before += (w < wid) ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int* __restrict__ in, int n,
+                                                              int* __restrict__ part) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        s += (i < n) ? in[i] : 0;
+    }
+    __shared__ int wsum[kScanThreads / 64];
+    s = wave_sum_i32(s);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += wsum[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of one tile per block, each tile offset by tile_prefix
+// (null: 0).  When total_out is given, block 0 of a single-tile launch
+// writes the grand total there.
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const int* __restrict__ in, int n,
+                                                            const int* __restrict__ tile_prefix,
+                                                            int* __restrict__ out,
+                                                            int* __restrict__ total_out) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_excl_scan(s, total) + (tile_prefix ? tile_prefix[blockIdx.x] : 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (total_out && threadIdx.x == 0) *total_out = total + (tile_prefix ? tile_prefix[blockIdx.x] : 0);
+}
+
+__global__ void k_scan_total(const int* __restrict__ src, int* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
+// Workspace per recursion level with t tiles: tile sums (t) followed by
+// their exclusive scan (t + 1, the last entry being the level total).
+size_t scan_workspace_ints(int n) {
+    size_t ws = 0;
+    int m = n;
+    while (m > kScanTile) {
+        const int tiles = (m + kScanTile - 1) / kScanTile;
+        ws += 2 * (size_t)tiles + 1;
+        m = tiles;
+    }
+    return ws + 1;
+}
+
+// out has n+1 entries; out[n] = total.  ws holds scan_workspace_ints(n).
+hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st) {
+    if (n <= 0) {
+        k_fill_i32<<<1, 64, 0, st>>>(out, 1, 0);
+        return hipGetLastError();
+    }
+    if (n <= kScanTile) {
+        k_scan_down<<<1, kScanThreads, 0, st>>>(in, n, nullptr, out, out + n);
+        return hipGetLastError();
+    }
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    int* part = ws;
+    int* part_ex = ws + tiles;  // tiles + 1 entries
+    k_scan_reduce<<<tiles, kScanThreads, 0, st>>>(in, n, part);
+    hipError_t e = launch_scan(part, part_ex, tiles, ws + 2 * (size_t)tiles + 1, st);
+    if (e != hipSuccess) return e;
+    k_scan_down<<<tiles, kScanThreads, 0, st>>>(in, n, part_ex, out, nullptr);
+    k_scan_total<<<1, 64, 0, st>>>(part_ex + tiles, out + n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Uniform grid: cell hash + count (ugrid_particles_cs.glsl:90-103).  The
+// atomic return value is a provisional rank; the stable order is restored
+// by k_stable_reorder.
+// ---------------------------------------------------------------------------
+__global__ void k_hash_count(const float4* __restrict__ pos, int n, GridDesc g,
+                             int* __restrict__ key, int* __restrict__ trank,
+                             int* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    const int ix = cell_axis(p.x, g.gmin[0], g.cell, g.dims[0]);
+    const int iy = cell_axis(p.y, g.gmin[1], g.cell, g.dims[1]);
+    const int iz = cell_axis(p.z, g.gmin[2], g.cell, g.dims[2]);
+    const int c = lin_cell(g.dims, ix, iy, iz);
+    key[i] = c;
+    trank[i] = atomicAdd(&count[c], 1);
+}
+
+// InsertParticle (ugrid_particles_cs.glsl:110-117) into a provisional list.
+__global__ void k_scatter_tmp(const int* __restrict__ key, const int* __restrict__ trank,
+                              const int* __restrict__ offset, int n, int* __restrict__ tmp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tmp[offset[key[i]] + trank[i]] = i;
+}
+
+// Stable insertion (UniformGrid2D::Build order, UniformGrid2D.cpp:75-91):
+// the rank of slot s inside its cell is the number of members of the cell
+// with a smaller slot, counted over the provisional list.  The particle
+// state is moved to its sorted slot in the same pass.
+__global__ void k_stable_reorder(const int* __restrict__ key, const int* __restrict__ offset,
+                                 const int* __restrict__ tmp, int n, int* __restrict__ list,
+                                 const float4* __restrict__ pos_in,
+                                 const float4* __restrict__ vel_in,
+                                 const float4* __restrict__ force_in,
+                                 const int* __restrict__ id_in, float4* __restrict__ pos_out,
+                                 float4* __restrict__ vel_out, float4* __restrict__ force_out,
+                                 int* __restrict__ id_out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int c = key[s];
+    const int b = offset[c], e = offset[c + 1];
+    int r = 0;
+    for (int k = b; k < e; ++k) r += (tmp[k] < s) ? 1 : 0;
+    const int d = b + r;
+    list[d] = s;
+    if (pos_out) {
+        pos_out[d] = pos_in[s];
+        vel_out[d] = vel_in[s];
+        force_out[d] = force_in[s];
+        id_out[d] = id_in[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour iteration: 9 columns (dx outer, dz inner), each the contiguous
+// sorted range of cells iy-1..iy+1.  Same order as the oracle.
+// ---------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const DevParams& P,
+                                                   const int* __restrict__ offset, int ix, int iy,
+                                                   int iz, F&& f) {
+    const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.dims[1] - 1);
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int cx = ix + dx;
+        if (cx < 0 || cx >= P.dims[0]) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int cz = iz + dz;
+            if (cz < 0 || cz >= P.dims[2]) continue;
+            const int b = offset[lin_cell(P.dims, cx, ylo, cz)];
+            const int e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
+            for (int j = b; j < e; ++j) f(j);
+        }
+    }
+}
+
+__device__ __forceinline__ bool coupled(const DevParams& P, float y, float H) {
+    return !(P.band > 0.0f) || fabsf(y - H) < P.band;
+}
+
+// ---------------------------------------------------------------------------
+// Density / pressure (rho_pres_comp.glsl:49-81) + particle flux term.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __restrict__ pos,
+                                                 const float4* __restrict__ vel,
+                                                 const int* __restrict__ offset, int n,
+                                                 const float* __restrict__ H0,
+                                                 float* __restrict__ rho_w,
+                                                 float2* __restrict__ pw,
+                                                 float* __restrict__ flux) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = pos[i];
+    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
+    const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
+    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+    float acc = 0.0f;
+    for_each_candidate(P, offset, ix, iy, iz, [&](int j) {
+        const float4 pj = pos[j];
+        const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+        if (r2 < P.r2_lt) {
+            const float q = P.h2 - r2;
+            acc += q * q * q;
+        }
+    });
+    float rho = P.m_poly6 * acc;
+    float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
+    const float u = (pi.x - P.org[0]) * P.uvs[0];
+    const float v = (pi.z - P.org[1]) * P.uvs[1];
+    const float height = tex_sample(H0, P.W, P.H, u, v);
+    if (coupled(P, pi.y, height)) {
+        const float wf = height * rho;
+        pressure += wf;
+        rho += wf / P.k_rp;
+    }
+    rho = fmaxf(P.rho0, rho);
+    rho_w[i] = rho;
+    pw[i] = make_float2(pressure, 1.0f / rho);
+    float fl = 0.0f;
+    if (P.band > 0.0f) {
+        const float d = fabsf(pi.y - height);
+        if (d < P.band) {
+            const float q = d / P.band;
+            const float w = (1.0f - q * q) * (1.0f - q * q);
+            fl = P.m_rho0 * vel[i].y * w;
+        }
+    }
+    flux[i] = fl;
+}
+
+// ---------------------------------------------------------------------------
+// Forces (force_comp.glsl:62-139) fused with symplectic Euler + boundary
+// (integrate_comp.glsl:56-92, 135-178).  Reads the sorted copy B, writes
+// the state A in the same (sorted) slot order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_force_integrate(
+    DevParams P, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
+    int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
+    const float* __restrict__ H0, float4* __restrict__ pos_o, float4* __restrict__ vel_o,
+    float4* __restrict__ force_o, int* __restrict__ id_o, float* __restrict__ rho_o,
+    float* __restrict__ prs_o) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    const float p_i = pw[i].x;
+    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
+    const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
+    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+    const float m_spiky_half = P.m * P.spiky * 0.5f;
+    const float m_lap = P.m * P.lap;
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
+    for_each_candidate(P, offset, ix, iy, iz, [&](int j) {
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2_exact(dx, dy, dz);
+        if (r2 < P.r2_lt && j != i) {
+            // an exactly coincident pair (possible after boundary clamping)
+            // has no pressure direction: rinv = 0 drops that term
+            const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+            const float r = r2 * rinv;
+            const float hr = P.h - r;
+            const float2 q = pw[j];
+            const float s = m_spiky_half * (p_i + q.x) * q.y * hr * hr * rinv;
+            pfx -= s * dx;
+            pfy -= s * dy;
+            pfz -= s * dz;
+            const float4 vj = vel[j];
+            const float t = m_lap * q.y * hr;
+            vfx += t * (vj.x - vi.x);
+            vfy += t * (vj.y - vi.y);
+            vfz += t * (vj.z - vi.z);
+        }
+    });
+    float4 fp = force[i];
+    if (pi.y > P.crest) {
+        fp.x /= P.brk_mass;
+        fp.y /= P.brk_mass;
+        fp.z /= P.brk_mass;
+        fp.w /= P.brk_mass;
+        vfx *= P.brk_visc;
+        vfy *= P.brk_visc;
+        vfz *= P.brk_visc;
+    }
+    vfx *= P.visc;
+    vfy *= P.visc;
+    vfz *= P.visc;
+    const float u = (pi.x - P.org[0]) * P.uvs[0];
+    const float v = (pi.z - P.org[1]) * P.uvs[1];
+    const float height = tex_sample(H0, P.W, P.H, u, v);
+    float tqx = (pi.y * fp.z - pi.z * fp.y) * P.torque;
+    float tqy = (pi.z * fp.x - pi.x * fp.z) * P.torque;
+    float tqz = (pi.x * fp.y - pi.y * fp.x) * P.torque;
+    float drx = 0.f, dry = 0.f, drz = 0.f, wfx = 0.f, wfy = 0.f, wfz = 0.f;
+    if (coupled(P, pi.y, height)) {
+        // WaveVelocity (force_comp.glsl:117-128)
+        const float hx = tex_sample(H0, P.W, P.H, u + P.du, v);
+        const float hy = tex_sample(H0, P.W, P.H, u, v + P.du);
+        const float wvx = (hx - height) / dt, wvy = (hy - height) / dt, wvz = (hx - hy) / P.du;
+        drx = -P.drag * (vi.x - wvx);
+        dry = -P.drag * (vi.y - wvy);
+        drz = -P.drag * (vi.z - wvz);
+        // WaveNormal (force_comp.glsl:131-139) = (H(u+1)-H, H(v+1)-H, -1)
+        const float nx = tex_sample(H0, P.W, P.H, u + 1.0f, v) - height;
+        const float ny = tex_sample(H0, P.W, P.H, u, v + 1.0f) - height;
+        wfx = -height * nx * P.wforce;
+        wfy = -height * ny * P.wforce;
+        wfz = -height * -1.0f * P.wforce;
+    }
+    float rho = rho_w[i];
+    float prs = p_i;
+    float Fx = pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
+    float Fy = pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
+    float Fz = pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
+    float Fw = fp.w;
+
+    // integrate_comp.glsl:56-92
+    float nvx = vi.x + dt * (Fx / rho), nvy = vi.y + dt * (Fy / rho), nvz = vi.z + dt * (Fz / rho);
+    float npx = pi.x + dt * nvx, npy = pi.y + dt * nvy, npz = pi.z + dt * nvz;
+    const float damp = 1.0f - P.vdamp * dt;
+    nvx *= damp;
+    nvy *= damp;
+    nvz *= damp;
+    if (sqrtf(dist2_exact(nvx, nvy, nvz)) > P.foam_speed) {
+        Fx *= P.foam_f;
+        Fy *= P.foam_f;
+        Fz *= P.foam_f;
+        Fw *= P.foam_f;
+        rho *= P.foam_rho;
+        prs *= P.foam_p;
+        nvx *= P.foam_v;
+        nvy *= P.foam_v;
+        nvz *= P.foam_v;
+    }
+    const float th = tex_sample(H0, P.W, P.H, (npx - P.org[0]) * P.uvs[0],
+                                (npz - P.org[1]) * P.uvs[1]);
+    if (npy < th) npy = th - P.rp;
+    // CheckBoundary (integrate_comp.glsl:135-178)
+    if (npx < P.lo[0]) { npx = P.lo[0]; nvx *= -P.bounce; }
+    else if (npx > P.up[0]) { npx = P.up[0]; nvx *= -P.bounce; }
+    if (npy < P.lo[1]) { npy = P.lo[1]; nvy *= -P.bounce; }
+    else if (npy > P.up[1]) { npy = P.up[1]; nvy *= -P.bounce; }
+    if (npz < P.lo[2]) { npz = P.lo[2]; nvz *= -P.bounce; }
+    else if (npz > P.up[2]) { npz = P.up[2]; nvz *= -P.bounce; }
+    if (P.wake) {
+        if (npy > P.up[1] + P.ceil_extra) { npy = P.up[1] + P.ceil_extra; nvy *= -P.bounce; }
+    }
+    pos_o[i] = make_float4(npx, npy, npz, pi.w);
+    vel_o[i] = make_float4(nvx, nvy, nvz, vi.w);
+    force_o[i] = make_float4(Fx, Fy, Fz, Fw);
+    id_o[i] = id[i];
+    rho_o[i] = rho;
+    prs_o[i] = prs;
+}
+
+// ---------------------------------------------------------------------------
+// Particle -> wave flux per vertical column of grid cells.  Column (ix, iz)
+// is one contiguous sorted range; one wave64 sums it (lane-strided partial
+// sums, then a fixed xor tree: deterministic).  Output at [iz*nx + ix].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_column_flux(const int* __restrict__ offset,
+                                                     const float* __restrict__ flux, int nx,
+                                                     int ny, int nz, float* __restrict__ colflux) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ncol = nx * nz;
+    if (wave >= ncol) return;
+    const int b = offset[wave * ny], e = offset[(wave + 1) * ny];
+    float s = 0.0f;
+    for (int k = b + lane; k < e; k += 64) s += flux[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        const int ix = wave / nz, iz = wave - ix * nz;
+        colflux[iz * nx + ix] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave EVOLVE (wave_comp.glsl:171-206), lambda split per axis, clamped
+// neighbours, optional particle flux source, wake line.  Arithmetic is not
+// contracted so the result matches the oracle bit for bit.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_wave(DevParams P, const float* __restrict__ h0,
+                                              const float* __restrict__ h1,
+                                              float* __restrict__ out,
+                                              const float* __restrict__ colflux, float dt) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int W = P.W, H = P.H;
+    if (i >= W || j >= H) return;
+    const size_t c = (size_t)j * W + i;
+    const float C0 = h0[c];
+    const float C1 = h1[c];
+    const float nn = h0[(size_t)min(j + 1, H - 1) * W + i];
+    const float ss = h0[(size_t)max(j - 1, 0) * W + i];
+    const float ee = h0[(size_t)j * W + min(i + 1, W - 1)];
+    const float ww = h0[(size_t)j * W + max(i - 1, 0)];
+    float o = P.wc0 * C0 + P.lx * (ee + ww) + P.lz * (nn + ss) - P.wc1 * C1;
+    o *= P.atten;
+    if (colflux) {
+        const float gx = P.ax * (float)i + P.bx, gz = P.az * (float)j + P.bz;
+        const float S = bilerp(colflux, P.dims[0], P.dims[2], gx, gz);
+        o += P.kappa * dt * (S * P.inv_area);
+    }
+    if (P.wake && o > P.wake_thr && i == W / 2) o += P.wake_amt;
+    out[c] = o;
+}
+
+// InitWave (wave_comp.glsl:82-140): peak * smoothstep(5, 0, d), d = distance
+// to the nearest of 2 or 3 centres.
+__global__ void k_init_wave(float* __restrict__ out, int W, int H, int c0x, int c0y, int c1x,
+                            int c1y, int c2x, int c2y, int three, float peak) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    auto dist = [&](int cx, int cy) {
+        const float dx = (float)(i - cx), dy = (float)(j - cy);
+        return sqrtf(dx * dx + dy * dy);
+    };
+    float d = fminf(dist(c0x, c0y), dist(c1x, c1y));
+    if (three) d = fminf(d, dist(c2x, c2y));
+    float t = (d - 5.0f) / (0.0f - 5.0f);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    out[(size_t)j * W + i] = peak * (t * t * (3.0f - 2.0f * t));
+}
+
+// Non-finite detector for the optional per-step check.
+__global__ void k_check_finite(const float* __restrict__ a, size_t n, int* __restrict__ flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    int bad = 0;
+    for (; i < n; i += stride) bad |= !isfinite(a[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_fill_i32<<<blocks_for(n, 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* trank,
+                             int* count, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(pos, n, g, key, trank, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offset, int n,
+                              int* tmp, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_scatter_tmp<<<blocks_for(n, 256), 256, 0, st>>>(key, trank, offset, n, tmp);
+    return hipGetLastError();
+}
+
+hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
+                                 int* list, const ParticleArrays* in, ParticleArrays* out,
+                                 hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_stable_reorder<<<blocks_for(n, 256), 256, 0, st>>>(
+        key, offset, tmp, n, list, in ? in->pos : nullptr, in ? in->vel : nullptr,
+        in ? in->force : nullptr, in ? in->id : nullptr, out ? out->pos : nullptr,
+        out ? out->vel : nullptr, out ? out->force : nullptr, out ? out->id : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
+                          const float* H0, float* rho_w, float2* pw, float* flux,
+                          hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_density<<<blocks_for(n, 256), 256, 0, st>>>(P, B.pos, B.vel, offset, n, H0, rho_w, pw,
+                                                  flux);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
+                                  const int* offset, int n, const float* rho_w, const float2* pw,
+                                  const float* H0, ParticleArrays& A, float* rho, float* prs,
+                                  hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P, dt, B.pos, B.vel, B.force, B.id,
+                                                          offset, n, rho_w, pw, H0, A.pos, A.vel,
+                                                          A.force, A.id, rho, prs);
+    return hipGetLastError();
+}
+
+hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
+                              float* colflux, hipStream_t st) {
+    const long long ncol = (long long)dims[0] * dims[2];
+    if (ncol <= 0) return hipSuccess;
+    k_column_flux<<<blocks_for(ncol * 64, 256), 256, 0, st>>>(offset, flux, dims[0], dims[1],
+                                                              dims[2], colflux);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,
+                       const float* colflux, float dt, hipStream_t st) {
+    if (P.W <= 0 || P.H <= 0) return hipSuccess;
+    dim3 blk(64, 4);
+    dim3 grd((P.W + 63) / 64, (P.H + 3) / 4);
+    k_wave<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_wave(float* out, int W, int H, float type, hipStream_t st) {
+    if (W <= 0 || H <= 0) return hipSuccess;
+    int c0x, c0y, c1x, c1y, c2x = 0, c2y = 0, three = 1;
+    float peak;
+    if (type == 1.0f) {  // splash
+        c0x = (int)(0.25f * (float)W);
+        c0y = (int)(0.25f * (float)H);
+        c1x = (int)(0.75f * (float)W);
+        c1y = (int)(0.75f * (float)H);
+        three = 0;
+        peak = 0.5f;
+    } else if (type == 0.0f) {  // wave
+        c0x = (int)(0.25f * (float)W);
+        c0y = H;
+        c2x = (int)(0.5f * (float)W);
+        c2y = H;
+        c1x = (int)(0.75f * (float)W);
+        c1y = H;
+        peak = 1.0f;
+    } else {  // wake
+        c0x = (int)(0.5f * (float)W);
+        c0y = (int)(0.1f * (float)H);
+        c1x = c2x = c0x;
+        c1y = c2y = c0y;
+        peak = 0.1f;
+    }
+    dim3 blk(64, 4);
+    dim3 grd((W + 63) / 64, (H + 3) / 4);
+    k_init_wave<<<grd, blk, 0, st>>>(out, W, H, c0x, c0y, c1x, c1y, c2x, c2y, three, peak);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
+    k_check_finite<<<blocks, 256, 0, st>>>(a, n, flag);
+    return hipGetLastError();
+}
+
+}  // namespace cwa

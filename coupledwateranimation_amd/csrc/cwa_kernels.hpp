@@ -1,0 +1,50 @@
+// cwa_kernels.hpp — launch wrappers of the coupled-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "cwa_device.hpp"
+
+namespace cwa {
+
+struct GridDesc {
+    float gmin[3];
+    float cell;
+    int dims[3];
+};
+
+// One copy of the particle state (SoA, device).
+struct ParticleArrays {
+    float4* pos = nullptr;
+    float4* vel = nullptr;
+    float4* force = nullptr;
+    int* id = nullptr;
+};
+
+size_t scan_workspace_ints(int n);
+hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st);
+hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st);
+hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* trank,
+                             int* count, hipStream_t st);
+hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offset, int n,
+                              int* tmp, hipStream_t st);
+// in/out may be null: only the index list is produced.
+hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
+                                 int* list, const ParticleArrays* in, ParticleArrays* out,
+                                 hipStream_t st);
+hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
+                          const float* H0, float* rho_w, float2* pw, float* flux,
+                          hipStream_t st);
+hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
+                                  const int* offset, int n, const float* rho_w, const float2* pw,
+                                  const float* H0, ParticleArrays& A, float* rho, float* prs,
+                                  hipStream_t st);
+hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
+                              float* colflux, hipStream_t st);
+hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,
+                       const float* colflux, float dt, hipStream_t st);
+hipError_t launch_init_wave(float* out, int W, int H, float type, hipStream_t st);
+hipError_t launch_check_finite(const float* a, size_t n, int* flag, hipStream_t st);
+
+}  // namespace cwa

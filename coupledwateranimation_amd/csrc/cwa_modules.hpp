@@ -1,0 +1,241 @@
+// cwa_modules.hpp — host-side objects of the coupled step, shaped after the
+// reference's Module framework:
+//   Module                       Module.h:6-32 (virtual Init/Reinit/Compute,
+//                                static registry, sComputeAll)
+//   ParallelScan                 SphWave2D/ParallelScan.h (Init(input, n),
+//                                Compute())
+//   UniformGrid3D                Ugrid3D / UgridParticles3D
+//                                (UniformGridParticles3D.h:50-100, BuildGrid)
+//   HeightField2DTripleBuffered  StencilImage2DTripleBuffered.h:8-41
+//   SphParticles3D               the particle SSBO + rho_pres / force /
+//                                integrate programs of Main.cpp
+//   CoupledScene                 the idle() step (Main.cpp:530-562)
+// All device memory is owned here (RAII); kernels are in cwa_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <list>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cwa.h"
+#include "cwa_device.hpp"
+#include "cwa_kernels.hpp"
+
+namespace cwa {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Owning device allocation.
+template <class T>
+class DeviceBuffer {
+   public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) { resize(n); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), n_(o.n_) {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    void resize(size_t n) {
+        if (n == n_) return;
+        release();
+        if (n) {
+            void* q = nullptr;
+            const hipError_t e = hipMalloc(&q, n * sizeof(T));
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(CWA_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            }
+            p_ = static_cast<T*>(q);
+        }
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+
+   private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Module.h:6-32.  Construction registers, destruction unregisters;
+// sComputeAll runs Compute() in construction order.  The registry is
+// process-wide like the reference's; it is not thread-safe.
+class Module {
+   public:
+    Module();
+    virtual ~Module();
+    Module(const Module&) = delete;
+    Module& operator=(const Module&) = delete;
+    virtual void Init() {}
+    virtual void Reinit() {}
+    virtual void Compute() {}
+    static std::list<Module*>& sAllModules();
+    static void sInitAll();
+    static void sReinitAll();
+    static void sComputeAll();
+};
+
+// SphWave2D/ParallelScan.h: exclusive scan of an int buffer of n values
+// into an owned output of n+1 values (any n >= 0).
+class ParallelScan {
+   public:
+    void Init(const int* input, int n);
+    void Compute(hipStream_t st);
+    int* GetOutput() const { return out_.get(); }
+
+   private:
+    const int* in_ = nullptr;
+    int n_ = 0;
+    DeviceBuffer<int> out_, ws_;
+};
+
+// Ugrid3D / UgridParticles3D: per-step counting sort of particle slots by
+// cell with a stable (slot-ordered) insertion, optionally moving the
+// particle state into sorted order.
+class UniformGrid3D {
+   public:
+    void Init(const GridDesc& g, int max_particles);
+    // BuildGrid (UniformGridParticles3D.cpp:170-211) over pos[0..n).
+    void BuildGrid(const float4* pos, int n, const ParticleArrays* in, ParticleArrays* out,
+                   hipStream_t st, hipEvent_t* ev = nullptr);
+    const GridDesc& desc() const { return g_; }
+    int num_cells() const { return g_.dims[0] * g_.dims[1] * g_.dims[2]; }
+    int* key() const { return key_.get(); }
+    int* count() const { return count_.get(); }
+    int* offset() const { return offset_.get(); }
+    int* list() const { return list_.get(); }
+
+   private:
+    GridDesc g_{};
+    DeviceBuffer<int> key_, trank_, count_, tmp_, list_, offset_, ws_;
+};
+
+// StencilImage2DTripleBuffered: three fp32 fields rotated by PingPong.
+class HeightField2DTripleBuffered {
+   public:
+    static constexpr int MODE_INIT = CWA_MODE_INIT;
+    static constexpr int MODE_INIT_FROM_TEXTURE = CWA_MODE_INIT_FROM_TEXTURE;
+    static constexpr int MODE_EVOLVE = CWA_MODE_EVOLVE;
+
+    void Allocate(int w, int h);
+    // Reinit (.cpp:42-59): MODE_INIT twice with PingPong.
+    void ReinitMode(float wave_type, hipStream_t st);
+    // Upload both history levels (newest first).
+    void Upload(const float* h0, const float* h1, hipStream_t st);
+    // ReinitFromTexture (.cpp:61-77) on a host image.
+    void ReinitFromTexture(const float* tex, int tex_w, int tex_h, hipStream_t st);
+    // Compute (.cpp:79-95) without the PingPong.
+    void Evolve(const DevParams& P, const float* colflux, float dt, hipStream_t st);
+    // PingPong (.cpp:33-40): written image becomes read 0.
+    void PingPong();
+    float* GetReadImage(int i) const { return img_[read_[i]].get(); }
+    float* GetWriteImage() const { return img_[write_].get(); }
+    int width() const { return w_; }
+    int height() const { return h_; }
+
+   private:
+    int w_ = 0, h_ = 0;
+    int read_[2] = {0, 1};
+    int write_ = 2;
+    DeviceBuffer<float> img_[3];
+};
+
+// The particle state (A), its sorted reorder target (B), per-step work
+// arrays and the visible density / pressure.
+class SphParticles3D {
+   public:
+    void Allocate(int n);
+    void Upload(const float* pos, const float* vel, const float* force, hipStream_t st);
+    int n() const { return n_; }
+    ParticleArrays A, B;
+    float* rho() const { return rho_.get(); }
+    float* prs() const { return prs_.get(); }
+    float* rho_w() const { return rho_w_.get(); }
+    float2* pw() const { return pw_.get(); }
+    float* flux() const { return flux_.get(); }
+
+   private:
+    int n_ = 0;
+    DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
+    DeviceBuffer<int> id_[2];
+    DeviceBuffer<float> rho_, prs_, rho_w_, flux_;
+    DeviceBuffer<float2> pw_;
+};
+
+// Kernel slots of cwa_kernel_times.
+enum KernelSlot {
+    K_CLEAR = 0,
+    K_HASH,
+    K_SCAN,
+    K_SCATTER,
+    K_REORDER,
+    K_DENSITY,
+    K_FORCE,
+    K_COLFLUX,
+    K_WAVE,
+    K_FINITE,
+    K_NUM
+};
+const char* kernel_slot_name(int k);
+
+// The coupled scene is the Module: Compute() is one idle() step at the time
+// step set by SetTimeStep (the reference bakes dt into the shaders).
+class CoupledScene : public Module {
+   public:
+    CoupledScene(const cwa_params& p, const cwa_init& init);
+    ~CoupledScene() override;
+    using Module::Reinit;
+    void Compute() override { Step(dt_); }
+    void SetTimeStep(float dt) { dt_ = dt; }
+    void Reinit(const cwa_init& init);
+    void SetParams(const cwa_params& p);
+    const cwa_params& params() const { return params_; }
+    // idle() (Main.cpp:530-562).  events: optional [K_NUM][2] per step.
+    void Step(float dt, hipEvent_t (*events)[2] = nullptr);
+    void StepGraph(float dt, int nsteps);
+    void WaveOnly(float dt);
+    void WaveReinit();
+    void WaveReinitFromTexture(const float* tex, int tw, int th);
+    void CheckFinite();
+    void Readback(cwa_host_state* h);
+    hipStream_t stream() const { return st_; }
+
+    SphParticles3D particles;
+    UniformGrid3D grid;
+    HeightField2DTripleBuffered wave;
+    DeviceBuffer<float> colflux;
+
+   private:
+    void build_devparams();
+    void invalidate_graph();
+    cwa_params params_{};
+    DevParams dp_{};
+    hipStream_t st_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    float graph_dt_ = 0.0f;
+    float dt_ = 0.0f;
+    DeviceBuffer<int> flag_;
+};
+
+// Validation shared by create / set_params.
+void validate_params(const cwa_params& p);
+DevParams make_devparams(const cwa_params& p, int wave_w, int wave_h);
+
+}  // namespace cwa

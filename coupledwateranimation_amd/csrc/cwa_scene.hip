@@ -1,0 +1,884 @@
+// cwa_scene.hip — host objects of the coupled step and the C ABI (cwa.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/cwa.h"
+#include "cwa_modules.hpp"
+
+namespace cwa {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(CWA_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Module registry (Module.cpp:5-59)
+// ---------------------------------------------------------------------------
+Module::Module() { sAllModules().push_back(this); }
+Module::~Module() { sAllModules().remove(this); }
+std::list<Module*>& Module::sAllModules() {
+    static std::list<Module*> all;
+    return all;
+}
+void Module::sInitAll() {
+    for (Module* p : sAllModules()) p->Init();
+}
+void Module::sReinitAll() {
+    for (Module* p : sAllModules()) p->Reinit();
+}
+void Module::sComputeAll() {
+    for (Module* p : sAllModules()) p->Compute();
+}
+
+// ---------------------------------------------------------------------------
+// ParallelScan
+// ---------------------------------------------------------------------------
+void ParallelScan::Init(const int* input, int n) {
+    if (n < 0) throw Error(CWA_EINVAL, "scan: n < 0");
+    in_ = input;
+    n_ = n;
+    out_.resize((size_t)n + 1);
+    ws_.resize(scan_workspace_ints(n));
+}
+
+void ParallelScan::Compute(hipStream_t st) {
+    hip_check(launch_scan(in_, out_.get(), n_, ws_.get(), st), "scan");
+}
+
+// ---------------------------------------------------------------------------
+// UniformGrid3D
+// ---------------------------------------------------------------------------
+void UniformGrid3D::Init(const GridDesc& g, int max_particles) {
+    g_ = g;
+    const size_t nc = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+    const size_t np = (size_t)std::max(max_particles, 1);
+    key_.resize(np);
+    trank_.resize(np);
+    tmp_.resize(np);
+    list_.resize(np);
+    count_.resize(nc);
+    offset_.resize(nc + 1);
+    ws_.resize(scan_workspace_ints((int)nc));
+}
+
+void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in,
+                              ParticleArrays* out, hipStream_t st, hipEvent_t* ev) {
+    const int nc = num_cells();
+    auto mark = [&](int slot, int which) {
+        if (ev) hip_check(hipEventRecord(ev[2 * slot + which], st), "event");
+    };
+    mark(K_CLEAR, 0);
+    hip_check(launch_fill_i32(count_.get(), nc, 0, st), "clear counts");
+    mark(K_CLEAR, 1);
+    mark(K_HASH, 0);
+    hip_check(launch_hash_count(pos, n, g_, key_.get(), trank_.get(), count_.get(), st), "hash");
+    mark(K_HASH, 1);
+    mark(K_SCAN, 0);
+    hip_check(launch_scan(count_.get(), offset_.get(), nc, ws_.get(), st), "scan");
+    mark(K_SCAN, 1);
+    mark(K_SCATTER, 0);
+    hip_check(launch_scatter_tmp(key_.get(), trank_.get(), offset_.get(), n, tmp_.get(), st),
+              "scatter");
+    mark(K_SCATTER, 1);
+    mark(K_REORDER, 0);
+    hip_check(launch_stable_reorder(key_.get(), offset_.get(), tmp_.get(), n, list_.get(), in,
+                                    out, st),
+              "reorder");
+    mark(K_REORDER, 1);
+}
+
+// ---------------------------------------------------------------------------
+// HeightField2DTripleBuffered
+// ---------------------------------------------------------------------------
+void HeightField2DTripleBuffered::Allocate(int w, int h) {
+    w_ = w;
+    h_ = h;
+    for (auto& b : img_) b.resize((size_t)w * h);
+    read_[0] = 0;
+    read_[1] = 1;
+    write_ = 2;
+}
+
+void HeightField2DTripleBuffered::ReinitMode(float wave_type, hipStream_t st) {
+    for (int k = 0; k < 2; ++k) {
+        hip_check(launch_init_wave(GetWriteImage(), w_, h_, wave_type, st), "init wave");
+        PingPong();
+    }
+}
+
+void HeightField2DTripleBuffered::Upload(const float* h0, const float* h1, hipStream_t st) {
+    const size_t bytes = (size_t)w_ * h_ * sizeof(float);
+    if (bytes == 0) return;
+    if (h0)
+        hip_check(hipMemcpyAsync(GetReadImage(0), h0, bytes, hipMemcpyHostToDevice, st), "h0");
+    else
+        hip_check(hipMemsetAsync(GetReadImage(0), 0, bytes, st), "h0 zero");
+    if (h1)
+        hip_check(hipMemcpyAsync(GetReadImage(1), h1, bytes, hipMemcpyHostToDevice, st), "h1");
+    else
+        hip_check(hipMemcpyAsync(GetReadImage(1), GetReadImage(0), bytes, hipMemcpyDeviceToDevice,
+                                 st),
+                  "h1 copy");
+    hip_check(hipMemsetAsync(GetWriteImage(), 0, bytes, st), "write zero");
+    hip_check(hipStreamSynchronize(st), "upload sync");
+}
+
+void HeightField2DTripleBuffered::ReinitFromTexture(const float* tex, int tex_w, int tex_h,
+                                                    hipStream_t st) {
+    if (tex_w < 2 * w_ || tex_h < h_) throw Error(CWA_EINVAL, "init texture too small");
+    std::vector<float> f((size_t)w_ * h_);
+    for (int j = 0; j < h_; ++j)
+        for (int i = 0; i < w_; ++i) f[(size_t)j * w_ + i] = tex[(size_t)j * tex_w + 2 * i];
+    hip_check(hipMemcpyAsync(GetWriteImage(), f.data(), f.size() * sizeof(float),
+                             hipMemcpyHostToDevice, st),
+              "texture upload");
+    hip_check(hipStreamSynchronize(st), "texture sync");
+    PingPong();
+}
+
+void HeightField2DTripleBuffered::Evolve(const DevParams& P, const float* colflux, float dt,
+                                         hipStream_t st) {
+    hip_check(launch_wave(P, GetReadImage(0), GetReadImage(1), GetWriteImage(), colflux, dt, st),
+              "wave");
+}
+
+// The texture units of the reference rotate so that the image just written
+// is read as the newest and the newest becomes the previous
+// (StencilImage2DTripleBuffered.cpp:33-40 with SwapUnits); period 3.
+void HeightField2DTripleBuffered::PingPong() {
+    const int w = write_;
+    write_ = read_[1];
+    read_[1] = read_[0];
+    read_[0] = w;
+}
+
+// ---------------------------------------------------------------------------
+// SphParticles3D
+// ---------------------------------------------------------------------------
+void SphParticles3D::Allocate(int n) {
+    n_ = n;
+    const size_t m = (size_t)std::max(n, 1);
+    for (int k = 0; k < 2; ++k) {
+        pos_[k].resize(m);
+        vel_[k].resize(m);
+        force_[k].resize(m);
+        id_[k].resize(m);
+    }
+    rho_.resize(m);
+    prs_.resize(m);
+    rho_w_.resize(m);
+    flux_.resize(m);
+    pw_.resize(m);
+    A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
+    B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
+}
+
+void SphParticles3D::Upload(const float* pos, const float* vel, const float* force,
+                            hipStream_t st) {
+    if (n_ == 0) return;
+    const size_t b4 = (size_t)n_ * sizeof(float4);
+    hip_check(hipMemcpyAsync(A.pos, pos, b4, hipMemcpyHostToDevice, st), "pos");
+    if (vel)
+        hip_check(hipMemcpyAsync(A.vel, vel, b4, hipMemcpyHostToDevice, st), "vel");
+    else
+        hip_check(hipMemsetAsync(A.vel, 0, b4, st), "vel zero");
+    if (force)
+        hip_check(hipMemcpyAsync(A.force, force, b4, hipMemcpyHostToDevice, st), "force");
+    else
+        hip_check(hipMemsetAsync(A.force, 0, b4, st), "force zero");
+    std::vector<int> ids(n_);
+    std::iota(ids.begin(), ids.end(), 0);
+    hip_check(hipMemcpyAsync(A.id, ids.data(), (size_t)n_ * sizeof(int), hipMemcpyHostToDevice,
+                             st),
+              "id");
+    // Particle.extras = (rest rho, 0, ...) at init (Main.cpp:774)
+    hip_check(hipMemsetAsync(prs_.get(), 0, (size_t)n_ * sizeof(float), st), "prs");
+    hip_check(hipStreamSynchronize(st), "upload sync");
+}
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+const char* kernel_slot_name(int k) {
+    static const char* names[K_NUM] = {"clear",   "hash",           "scan",
+                                       "scatter", "reorder",        "density",
+                                       "force_integrate", "column_flux", "wave",
+                                       "finite"};
+    return (k >= 0 && k < K_NUM) ? names[k] : "?";
+}
+
+void validate_params(const cwa_params& p) {
+    auto bad = [](const char* m) { throw Error(CWA_EINVAL, m); };
+    if (!(p.h > 0.0f) || !std::isfinite(p.h)) bad("h must be > 0");
+    if (!(p.cell_size >= p.h)) bad("cell_size must be >= h (27-cell search)");
+    for (int a = 0; a < 3; ++a)
+        if (p.grid_dims[a] < 1) bad("grid_dims must be >= 1");
+    const double nc = (double)p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2];
+    if (nc > (double)(1 << 30)) bad("grid has more than 2^30 cells");
+    if (!(p.rest_rho > 0.0f)) bad("rest_rho must be > 0");
+    if (!(p.gas_const * p.particle_radius != 0.0f)) bad("gas_const * particle_radius must be != 0");
+    if (p.wave_uv_scale[0] == 0.0f || p.wave_uv_scale[1] == 0.0f) bad("wave_uv_scale must be != 0");
+    if (p.wave_vel_du == 0.0f) bad("wave_vel_du must be != 0");
+    if (p.breaking_mass == 0.0f) bad("breaking_mass must be != 0");
+}
+
+// Smallest float x with sqrtf(x) >= h, so that  sqrt_rn(r2) < h  <=>  r2 < x
+// (sqrt_rn is monotone).  Host sqrtf is IEEE correctly rounded.
+static float exact_r2_threshold(float h) {
+    float x = h * h;
+    while (x > 0.0f && std::sqrt(x) >= h) x = std::nextafter(x, 0.0f);
+    while (!(std::sqrt(x) >= h)) x = std::nextafter(x, INFINITY);
+    return x;
+}
+
+DevParams make_devparams(const cwa_params& p, int W, int H) {
+    DevParams d{};
+    const float pi = 3.141592741f;  // PI as written in the shaders
+    d.h = p.h;
+    d.h2 = p.h * p.h;
+    d.r2_lt = exact_r2_threshold(p.h);
+    d.m = p.mass;
+    const float h2 = p.h * p.h, h6 = h2 * h2 * h2, h9 = h6 * h2 * p.h;
+    const float poly6 = 315.0f / (64.0f * pi * h9);
+    d.m_poly6 = p.mass * poly6;
+    d.rho0 = p.rest_rho;
+    d.k = p.gas_const;
+    d.k_rp = p.gas_const * p.particle_radius;
+    d.rp = p.particle_radius;
+    d.spiky = -p.spiky_coeff / (pi * h6);
+    d.lap = p.visc_coeff / (pi * h6);
+    d.visc = p.visc;
+    for (int a = 0; a < 3; ++a) d.g[a] = p.gravity[a];
+    d.crest = p.crest_threshold;
+    d.brk_mass = p.breaking_mass;
+    d.brk_visc = p.breaking_visc;
+    d.torque = p.torque_coeff;
+    d.drag = p.drag_coeff;
+    d.wforce = p.wave_force_coeff;
+    d.du = p.wave_vel_du;
+    d.vdamp = p.vel_damping;
+    d.bounce = p.bounce;
+    d.foam_speed = p.foam_speed;
+    d.foam_f = p.foam_force;
+    d.foam_rho = p.foam_rho;
+    d.foam_p = p.foam_pressure;
+    d.foam_v = p.foam_vel;
+    for (int a = 0; a < 3; ++a) {
+        d.up[a] = p.upper[a];
+        d.lo[a] = p.lower[a];
+        d.gmin[a] = p.grid_min[a];
+        d.dims[a] = p.grid_dims[a];
+    }
+    d.ceil_extra = p.ceiling_extra;
+    d.wake = (p.wave_type > 0.0f && p.wave_type < 1.0f) ? 1 : 0;
+    for (int a = 0; a < 2; ++a) {
+        d.org[a] = p.wave_origin[a];
+        d.uvs[a] = p.wave_uv_scale[a];
+    }
+    d.W = W;
+    d.H = H;
+    d.wc0 = 2.0f - 2.0f * p.lambda_x - 2.0f * p.lambda_z - p.beta;
+    d.wc1 = 1.0f - p.beta;
+    d.lx = p.lambda_x;
+    d.lz = p.lambda_z;
+    d.atten = p.atten;
+    d.wake_thr = p.wake_threshold;
+    d.wake_amt = p.wake_amount;
+    d.band = p.couple_band;
+    d.kappa = p.couple_strength;
+    d.m_rho0 = p.mass / p.rest_rho;
+    d.cell = p.cell_size;
+    d.inv_area = 1.0f / (p.cell_size * p.cell_size);
+    // texel -> column-grid coordinates (computed in double, rounded once)
+    const double Wd = W > 0 ? W : 1, Hd = H > 0 ? H : 1, cell = p.cell_size;
+    d.ax = (float)(1.0 / (Wd * (double)p.wave_uv_scale[0] * cell));
+    d.bx = (float)(((double)p.wave_origin[0] + 0.5 / (Wd * (double)p.wave_uv_scale[0]) -
+                    (double)p.grid_min[0]) / cell - 0.5);
+    d.az = (float)(1.0 / (Hd * (double)p.wave_uv_scale[1] * cell));
+    d.bz = (float)(((double)p.wave_origin[1] + 0.5 / (Hd * (double)p.wave_uv_scale[1]) -
+                    (double)p.grid_min[2]) / cell - 0.5);
+    d.num_cells = p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2];
+    return d;
+}
+
+static GridDesc grid_desc(const cwa_params& p) {
+    GridDesc g{};
+    for (int a = 0; a < 3; ++a) {
+        g.gmin[a] = p.grid_min[a];
+        g.dims[a] = p.grid_dims[a];
+    }
+    g.cell = p.cell_size;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// CoupledScene
+// ---------------------------------------------------------------------------
+static void validate_init(const cwa_init& in) {
+    if (in.n < 0) throw Error(CWA_EINVAL, "n < 0");
+    if (in.n > 0 && !in.pos) throw Error(CWA_EINVAL, "pos is null");
+    if (in.wave_w < 1 || in.wave_h < 1) throw Error(CWA_EINVAL, "wave size must be >= 1x1");
+}
+
+CoupledScene::CoupledScene(const cwa_params& p, const cwa_init& init) {
+    validate_params(p);
+    validate_init(init);
+    hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
+    params_ = p;
+    flag_.resize(1);
+    Reinit(init);
+}
+
+CoupledScene::~CoupledScene() {
+    invalidate_graph();
+    if (st_) {
+        (void)hipStreamSynchronize(st_);
+        (void)hipStreamDestroy(st_);
+    }
+}
+
+void CoupledScene::invalidate_graph() {
+    if (gexec_) {
+        (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+    }
+}
+
+void CoupledScene::build_devparams() { dp_ = make_devparams(params_, wave.width(), wave.height()); }
+
+void CoupledScene::Reinit(const cwa_init& init) {
+    validate_init(init);
+    invalidate_graph();
+    hip_check(hipStreamSynchronize(st_), "sync");
+    particles.Allocate(init.n);
+    particles.Upload(init.pos, init.vel, init.force, st_);
+    grid.Init(grid_desc(params_), init.n);
+    hip_check(launch_fill_i32(grid.count(), grid.num_cells(), 0, st_), "count zero");
+    hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
+    colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
+    wave.Allocate(init.wave_w, init.wave_h);
+    wave.Upload(init.height0, init.height1, st_);
+    build_devparams();
+    hip_check(hipStreamSynchronize(st_), "reinit sync");
+}
+
+void CoupledScene::SetParams(const cwa_params& p) {
+    validate_params(p);
+    const bool grid_changed = p.grid_dims[0] != params_.grid_dims[0] ||
+                              p.grid_dims[1] != params_.grid_dims[1] ||
+                              p.grid_dims[2] != params_.grid_dims[2] ||
+                              p.grid_min[0] != params_.grid_min[0] ||
+                              p.grid_min[1] != params_.grid_min[1] ||
+                              p.grid_min[2] != params_.grid_min[2] ||
+                              p.cell_size != params_.cell_size;
+    invalidate_graph();
+    hip_check(hipStreamSynchronize(st_), "sync");
+    params_ = p;
+    if (grid_changed) {
+        grid.Init(grid_desc(params_), particles.n());
+        hip_check(launch_fill_i32(grid.count(), grid.num_cells(), 0, st_), "count zero");
+        hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
+        colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
+    }
+    build_devparams();
+}
+
+void CoupledScene::Step(float dt, hipEvent_t (*ev)[2]) {
+    auto mark = [&](int slot, int which) {
+        if (ev) hip_check(hipEventRecord(ev[slot][which], st_), "event");
+    };
+    const int n = particles.n();
+    const float* H0 = wave.GetReadImage(0);
+    const bool couple = n > 0 && params_.couple_band > 0.0f;
+    if (n > 0) {
+        grid.BuildGrid(particles.A.pos, n, &particles.A, &particles.B, st_,
+                       ev ? &ev[0][0] : nullptr);
+        mark(K_DENSITY, 0);
+        hip_check(launch_density(dp_, particles.B, grid.offset(), n, H0, particles.rho_w(),
+                                 particles.pw(), particles.flux(), st_),
+                  "density");
+        mark(K_DENSITY, 1);
+        mark(K_FORCE, 0);
+        hip_check(launch_force_integrate(dp_, dt, particles.B, grid.offset(), n, particles.rho_w(),
+                                         particles.pw(), H0, particles.A, particles.rho(),
+                                         particles.prs(), st_),
+                  "force/integrate");
+        mark(K_FORCE, 1);
+        if (couple) {
+            mark(K_COLFLUX, 0);
+            hip_check(launch_column_flux(grid.offset(), particles.flux(), params_.grid_dims,
+                                         colflux.get(), st_),
+                      "column flux");
+            mark(K_COLFLUX, 1);
+        }
+    }
+    mark(K_WAVE, 0);
+    wave.Evolve(dp_, couple ? colflux.get() : nullptr, dt, st_);
+    mark(K_WAVE, 1);
+    wave.PingPong();
+    if (params_.check_finite && !ev) CheckFinite();
+}
+
+void CoupledScene::CheckFinite() {
+    hip_check(launch_fill_i32(flag_.get(), 1, 0, st_), "flag");
+    const int n = particles.n();
+    if (n > 0) {
+        hip_check(launch_check_finite(reinterpret_cast<const float*>(particles.A.pos),
+                                      4 * (size_t)n, flag_.get(), st_),
+                  "finite pos");
+        hip_check(launch_check_finite(reinterpret_cast<const float*>(particles.A.vel),
+                                      4 * (size_t)n, flag_.get(), st_),
+                  "finite vel");
+    }
+    hip_check(launch_check_finite(wave.GetReadImage(0), (size_t)wave.width() * wave.height(),
+                                  flag_.get(), st_),
+              "finite height");
+    int h = 0;
+    hip_check(hipMemcpyAsync(&h, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, st_), "flag");
+    hip_check(hipStreamSynchronize(st_), "flag sync");
+    if (h) throw Error(CWA_ENONFINITE, "non-finite particle or height value after step");
+}
+
+void CoupledScene::StepGraph(float dt, int nsteps) {
+    if (params_.check_finite) {  // the finite check synchronises: run eagerly
+        for (int k = 0; k < nsteps; ++k) Step(dt);
+        return;
+    }
+    const int cycle = 3;  // wave triple buffer period (particles: period 1)
+    if (nsteps >= cycle && (!gexec_ || graph_dt_ != dt)) {
+        invalidate_graph();
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+        try {
+            for (int k = 0; k < cycle; ++k) Step(dt);
+        } catch (...) {
+            (void)hipStreamEndCapture(st_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st_, &g), "end capture");
+        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "graph instantiate");
+        graph_dt_ = dt;
+    }
+    int done = 0;
+    while (gexec_ && nsteps - done >= cycle) {
+        hip_check(hipGraphLaunch(gexec_, st_), "graph launch");
+        done += cycle;
+    }
+    for (; done < nsteps; ++done) Step(dt);
+}
+
+void CoupledScene::WaveOnly(float dt) {
+    wave.Evolve(dp_, nullptr, dt, st_);
+    wave.PingPong();
+}
+
+void CoupledScene::WaveReinit() {
+    invalidate_graph();
+    wave.ReinitMode(params_.wave_type, st_);
+    hip_check(hipStreamSynchronize(st_), "sync");
+}
+
+void CoupledScene::WaveReinitFromTexture(const float* tex, int tw, int th) {
+    invalidate_graph();
+    hip_check(hipStreamSynchronize(st_), "sync");
+    wave.ReinitFromTexture(tex, tw, th, st_);
+}
+
+void CoupledScene::Readback(cwa_host_state* h) {
+    const int n = particles.n();
+    hip_check(hipStreamSynchronize(st_), "sync");
+    if (n > 0) {
+        std::vector<int> id(n);
+        hip_check(hipMemcpy(id.data(), particles.A.id, n * sizeof(int), hipMemcpyDeviceToHost),
+                  "id");
+        auto un4 = [&](const float4* d, float* out) {
+            if (!out) return;
+            std::vector<float4> t(n);
+            hip_check(hipMemcpy(t.data(), d, n * sizeof(float4), hipMemcpyDeviceToHost), "rb4");
+            for (int s = 0; s < n; ++s) std::memcpy(out + 4 * (size_t)id[s], &t[s], sizeof(float4));
+        };
+        auto un1 = [&](const float* d, float* out) {
+            if (!out) return;
+            std::vector<float> t(n);
+            hip_check(hipMemcpy(t.data(), d, n * sizeof(float), hipMemcpyDeviceToHost), "rb1");
+            for (int s = 0; s < n; ++s) out[id[s]] = t[s];
+        };
+        un4(particles.A.pos, h->pos);
+        un4(particles.A.vel, h->vel);
+        un4(particles.A.force, h->force);
+        un1(particles.rho(), h->rho);
+        un1(particles.prs(), h->prs);
+    }
+    const size_t bytes = (size_t)wave.width() * wave.height() * sizeof(float);
+    if (h->height0)
+        hip_check(hipMemcpy(h->height0, wave.GetReadImage(0), bytes, hipMemcpyDeviceToHost), "h0");
+    if (h->height1)
+        hip_check(hipMemcpy(h->height1, wave.GetReadImage(1), bytes, hipMemcpyDeviceToHost), "h1");
+}
+
+}  // namespace cwa
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace cwa;
+
+struct cwa_scene {
+    CoupledScene* impl;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return CWA_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return CWA_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CWA_EINVAL;
+    }
+}
+
+#define CWA_REQUIRE(cond, msg)                  \
+    do {                                        \
+        if (!(cond)) throw Error(CWA_EINVAL, msg); \
+    } while (0)
+
+extern "C" {
+
+int cwa_abi_version(void) { return CWA_ABI_VERSION; }
+
+int cwa_struct_size(int which) {
+    switch (which) {
+        case 0: return (int)sizeof(cwa_params);
+        case 1: return (int)sizeof(cwa_init);
+        case 2: return (int)sizeof(cwa_particle_view);
+        case 3: return (int)sizeof(cwa_grid_view);
+        case 4: return (int)sizeof(cwa_host_state);
+        case 5: return (int)sizeof(cwa_kernel_times);
+        default: return -1;
+    }
+}
+
+const char* cwa_last_error(void) { return g_last_error.c_str(); }
+
+int cwa_device_count(void) {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_last_error = hipGetErrorString(e);
+        return CWA_EHIP;
+    }
+    return n;
+}
+
+const char* cwa_kernel_name(int k) { return kernel_slot_name(k); }
+
+void cwa_params_reference(cwa_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // ConstantsUniform (Main.cpp:184-190)
+    p->mass = 0.02f;
+    p->h = 2.0f * 0.005f;  // smoothing_coeff * PARTICLE_RADIUS
+    p->visc = 3000.0f;
+    p->rest_rho = 1000.0f;
+    p->gas_const = 4000.0f;       // rho_pres_comp.glsl:41
+    p->particle_radius = 0.005f;  // PARTICLE_RADIUS
+    p->spiky_coeff = 20.0f;       // force_comp.glsl:68
+    p->visc_coeff = 20.0f;        // laplacian = -spiky, force_comp.glsl:69
+    p->gravity[0] = 0.0f;
+    p->gravity[1] = -9806.65f;  // force_comp.glsl:52
+    p->gravity[2] = 0.0f;
+    p->crest_threshold = 0.01f;  // force_comp.glsl:5
+    p->breaking_mass = 0.25f;
+    p->breaking_visc = 0.5f;
+    p->torque_coeff = 0.25f;
+    p->drag_coeff = 0.25f;
+    p->wave_force_coeff = 0.5f;
+    p->wave_vel_du = 0.01f;
+    p->vel_damping = 0.3f;  // integrate_comp.glsl:8
+    p->bounce = 0.3f;
+    p->foam_speed = 25.0f;
+    p->foam_force = 0.5f;
+    p->foam_rho = 0.1f;
+    p->foam_pressure = 0.25f;
+    p->foam_vel = 0.1f;
+    // BoundaryUniform (Main.cpp:192-198)
+    p->upper[0] = 0.48f;
+    p->upper[1] = 1.0f;
+    p->upper[2] = 0.48f;
+    p->lower[0] = 0.0f;
+    p->lower[1] = -0.02f;
+    p->lower[2] = 0.0f;
+    p->ceiling_extra = 0.1f;
+    p->wave_origin[0] = 0.0f;
+    p->wave_origin[1] = 0.0f;
+    p->wave_uv_scale[0] = 2.0f;  // coord = 2*pos.xz
+    p->wave_uv_scale[1] = 2.0f;
+    // WaveUniforms (Main.cpp:200-204)
+    p->lambda_x = 0.01f;
+    p->lambda_z = 0.01f;
+    p->atten = 0.985f;
+    p->beta = 0.001f;
+    p->wave_type = 1.0f;
+    p->wake_threshold = 0.0001f;
+    p->wake_amount = 0.001f;
+    p->couple_band = 0.0f;
+    p->couple_strength = 0.0f;
+    // grid over the box [lower, upper], cell = h
+    p->cell_size = p->h;
+    for (int a = 0; a < 3; ++a) {
+        p->grid_min[a] = p->lower[a];
+        p->grid_dims[a] =
+            std::max(1, (int)std::ceil((p->upper[a] - p->lower[a]) / p->cell_size - 1e-4f));
+    }
+    p->check_finite = 0;
+}
+
+int cwa_scene_create(const cwa_params* p, const cwa_init* init, cwa_scene** out) {
+    return guarded([&] {
+        CWA_REQUIRE(p && init && out, "null argument");
+        *out = nullptr;
+        auto* s = new cwa_scene{nullptr};
+        try {
+            s->impl = new CoupledScene(*p, *init);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int cwa_scene_reinit(cwa_scene* s, const cwa_init* init) {
+    return guarded([&] {
+        CWA_REQUIRE(s && init, "null argument");
+        s->impl->Reinit(*init);
+    });
+}
+
+int cwa_scene_destroy(cwa_scene* s) {
+    return guarded([&] {
+        if (!s) return;
+        delete s->impl;
+        delete s;
+    });
+}
+
+int cwa_scene_set_params(cwa_scene* s, const cwa_params* p) {
+    return guarded([&] {
+        CWA_REQUIRE(s && p, "null argument");
+        s->impl->SetParams(*p);
+    });
+}
+
+int cwa_scene_get_params(const cwa_scene* s, cwa_params* p) {
+    return guarded([&] {
+        CWA_REQUIRE(s && p, "null argument");
+        *p = s->impl->params();
+    });
+}
+
+int cwa_scene_wave_reinit(cwa_scene* s) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        s->impl->WaveReinit();
+    });
+}
+
+int cwa_scene_wave_reinit_from_texture(cwa_scene* s, const float* tex, int tex_w, int tex_h) {
+    return guarded([&] {
+        CWA_REQUIRE(s && tex, "null argument");
+        s->impl->WaveReinitFromTexture(tex, tex_w, tex_h);
+    });
+}
+
+int cwa_scene_wave_compute(cwa_scene* s, float dt) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        s->impl->WaveOnly(dt);
+    });
+}
+
+int cwa_scene_height(const cwa_scene* s, int age, float** dptr, int* w, int* h, int* pitch) {
+    return guarded([&] {
+        CWA_REQUIRE(s && dptr, "null argument");
+        CWA_REQUIRE(age >= 0 && age <= 2, "age must be 0, 1 or 2");
+        const auto& wv = s->impl->wave;
+        *dptr = age == 2 ? wv.GetWriteImage() : wv.GetReadImage(age);
+        if (w) *w = wv.width();
+        if (h) *h = wv.height();
+        if (pitch) *pitch = wv.width();
+    });
+}
+
+int cwa_scene_step(cwa_scene* s, float dt, int nsteps) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        CWA_REQUIRE(nsteps >= 0, "nsteps < 0");
+        CWA_REQUIRE(std::isfinite(dt), "dt must be finite");
+        for (int k = 0; k < nsteps; ++k) s->impl->Step(dt);
+    });
+}
+
+int cwa_scene_step_graph(cwa_scene* s, float dt, int nsteps) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        CWA_REQUIRE(nsteps >= 0, "nsteps < 0");
+        CWA_REQUIRE(std::isfinite(dt), "dt must be finite");
+        s->impl->StepGraph(dt, nsteps);
+    });
+}
+
+int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times* out) {
+    return guarded([&] {
+        CWA_REQUIRE(s && out, "null argument");
+        CWA_REQUIRE(nsteps >= 0, "nsteps < 0");
+        std::memset(out, 0, sizeof(*out));
+        hipEvent_t ev[K_NUM][2];
+        for (auto& e : ev)
+            for (auto& x : e) hip_check(hipEventCreate(&x), "event create");
+        struct Cleanup {
+            hipEvent_t (*e)[2];
+            ~Cleanup() {
+                for (int k = 0; k < K_NUM; ++k)
+                    for (int q = 0; q < 2; ++q) (void)hipEventDestroy(e[k][q]);
+            }
+        } cleanup{ev};
+        CoupledScene& S = *s->impl;
+        const bool couple = S.particles.n() > 0 && S.params().couple_band > 0.0f;
+        for (int k = 0; k < nsteps; ++k) {
+            S.Step(dt, ev);
+            hip_check(hipStreamSynchronize(S.stream()), "sync");
+            for (int slot = 0; slot < K_NUM; ++slot) {
+                const bool ran = slot == K_WAVE ||
+                                 (S.particles.n() > 0 && slot <= K_FORCE) ||
+                                 (slot == K_COLFLUX && couple);
+                if (!ran) continue;
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, ev[slot][0], ev[slot][1]), "elapsed");
+                out->ms[slot] += ms;
+                out->launches[slot] += 1;
+            }
+        }
+        for (int slot = 0; slot < K_NUM; ++slot)
+            if (out->launches[slot]) out->ms[slot] /= (float)out->launches[slot];
+        out->steps = nsteps;
+    });
+}
+
+int cwa_scene_sync(cwa_scene* s) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        hip_check(hipStreamSynchronize(s->impl->stream()), "sync");
+    });
+}
+
+void* cwa_scene_stream(const cwa_scene* s) { return s ? (void*)s->impl->stream() : nullptr; }
+
+int cwa_scene_particles(const cwa_scene* s, cwa_particle_view* v) {
+    return guarded([&] {
+        CWA_REQUIRE(s && v, "null argument");
+        const auto& P = s->impl->particles;
+        v->n = P.n();
+        v->pos = reinterpret_cast<float*>(P.A.pos);
+        v->vel = reinterpret_cast<float*>(P.A.vel);
+        v->force = reinterpret_cast<float*>(P.A.force);
+        v->rho = P.rho();
+        v->prs = P.prs();
+        v->id = P.A.id;
+    });
+}
+
+int cwa_scene_grid(const cwa_scene* s, cwa_grid_view* v) {
+    return guarded([&] {
+        CWA_REQUIRE(s && v, "null argument");
+        const auto& G = s->impl->grid;
+        for (int a = 0; a < 3; ++a) v->dims[a] = G.desc().dims[a];
+        v->num_cells = G.num_cells();
+        v->n = s->impl->particles.n();
+        v->key = G.key();
+        v->count = G.count();
+        v->offset = G.offset();
+        v->list = G.list();
+        v->flux = s->impl->colflux.get();
+    });
+}
+
+int cwa_scene_readback(cwa_scene* s, cwa_host_state* h) {
+    return guarded([&] {
+        CWA_REQUIRE(s && h, "null argument");
+        s->impl->Readback(h);
+    });
+}
+
+int cwa_scan_exclusive_i32(const int32_t* d_in, int32_t* d_out, int n, void* stream) {
+    return guarded([&] {
+        CWA_REQUIRE(n >= 0, "n < 0");
+        CWA_REQUIRE(d_out && (n == 0 || d_in), "null argument");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        DeviceBuffer<int> ws(scan_workspace_ints(n));
+        hip_check(launch_scan(d_in, d_out, n, ws.get(), st), "scan");
+        hip_check(hipStreamSynchronize(st), "scan sync");
+    });
+}
+
+int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float cell_size,
+                   const int32_t dims[3], int32_t* d_key, int32_t* d_count, int32_t* d_offset,
+                   int32_t* d_list, void* stream) {
+    return guarded([&] {
+        CWA_REQUIRE(n >= 0, "n < 0");
+        CWA_REQUIRE(grid_min && dims && d_count && d_offset, "null argument");
+        CWA_REQUIRE(n == 0 || (d_pos4 && d_key && d_list), "null argument");
+        CWA_REQUIRE(cell_size > 0.0f, "cell_size must be > 0");
+        CWA_REQUIRE(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, "dims must be >= 1");
+        CWA_REQUIRE((double)dims[0] * dims[1] * dims[2] <= (double)(1 << 30), "too many cells");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        GridDesc g{};
+        for (int a = 0; a < 3; ++a) {
+            g.gmin[a] = grid_min[a];
+            g.dims[a] = dims[a];
+        }
+        g.cell = cell_size;
+        const int nc = dims[0] * dims[1] * dims[2];
+        DeviceBuffer<int> trank((size_t)std::max(n, 1)), tmp((size_t)std::max(n, 1)),
+            ws(scan_workspace_ints(nc));
+        hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
+        hip_check(launch_hash_count(reinterpret_cast<const float4*>(d_pos4), n, g, d_key,
+                                    trank.get(), d_count, st),
+                  "hash");
+        hip_check(launch_scan(d_count, d_offset, nc, ws.get(), st), "scan");
+        hip_check(launch_scatter_tmp(d_key, trank.get(), d_offset, n, tmp.get(), st), "scatter");
+        hip_check(launch_stable_reorder(d_key, d_offset, tmp.get(), n, d_list, nullptr, nullptr,
+                                        st),
+                  "stable insert");
+        hip_check(hipStreamSynchronize(st), "grid sync");
+    });
+}
+
+}  // extern "C"

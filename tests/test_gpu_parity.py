@@ -1,0 +1,349 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars (stated per test):
+  scan, cell keys, counts, offsets, stable list ... bit-exact
+  wave stencil, InitWave (no particles) ............ bit-exact (both sides
+                                                     round every operation
+                                                     as written)
+  particle fields after 1 step ..................... fp32 relative tolerance
+  particle fields after N steps .................... error against the FP64
+                                                     oracle within a stated
+                                                     multiple of the FP32
+                                                     oracle's own error
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).with_name("golden") / "scan_kat.json"
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ----------------------------------------------------------------- scan
+def _gpu_scan(cwa, torch, dev, x):
+    t = torch.tensor(np.asarray(x, np.int32), device=dev)
+    out = torch.full((len(x) + 1,), -7, dtype=torch.int32, device=dev)
+    from coupledwateranimation_amd import _lib
+
+    _lib.check(cwa.lib().cwa_scan_exclusive_i32(t.data_ptr() if len(x) else None,
+                                                out.data_ptr(), len(x), None))
+    return out.cpu().numpy()
+
+
+def test_scan_kat_gpu(gpu, cwa, oracle):
+    import torch
+
+    kat = json.loads(GOLD.read_text())
+    for case in kat.values():
+        out = _gpu_scan(cwa, torch, gpu, case["input"])
+        assert out[:-1].tolist() == case["expected_exclusive"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 2047, 2048, 2049, 100_003, 4_194_305, 9_000_001])
+def test_scan_sizes_gpu(gpu, cwa, oracle, n):
+    import torch
+
+    _say(f"scan n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 40, n).astype(np.int32)
+    out = _gpu_scan(cwa, torch, gpu, x)
+    assert np.array_equal(out, oracle.exclusive_scan(x))
+
+
+# ----------------------------------------------------------------- grid
+def _gpu_grid(cwa, torch, dev, p, pos):
+    import ctypes as C
+    from coupledwateranimation_amd import _lib
+
+    n = pos.shape[0]
+    nc = p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2]
+    dpos = torch.tensor(pos, device=dev)
+    key = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    lst = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    cnt = torch.empty(nc, dtype=torch.int32, device=dev)
+    off = torch.empty(nc + 1, dtype=torch.int32, device=dev)
+    gmin = (C.c_float * 3)(*p.grid_min)
+    dims = (C.c_int32 * 3)(*p.grid_dims)
+    _lib.check(cwa.lib().cwa_grid_build(dpos.data_ptr() if n else None, n, gmin, p.cell_size,
+                                        dims, key.data_ptr(), cnt.data_ptr(), off.data_ptr(),
+                                        lst.data_ptr(), None))
+    return {"key": key.cpu().numpy()[:n], "count": cnt.cpu().numpy(),
+            "offset": off.cpu().numpy(), "list": lst.cpu().numpy()[:n]}
+
+
+def _grid_cases(cwa):
+    from coupledwateranimation_amd import configs
+
+    out = []
+    s = configs.dam_break_small()
+    out.append(("dam_break_4k", s.params, s.pos))
+    rng = np.random.default_rng(11)
+    p = cwa.reference_params()
+    pos = np.ones((20000, 4), np.float32)
+    pos[:, :3] = rng.uniform(-0.2, 1.3, (20000, 3))          # many outside the grid
+    out.append(("outside", p, pos))
+    pos = np.ones((3000, 4), np.float32) * 0.1234            # all in one cell
+    out.append(("one_cell", p, pos))
+    out.append(("empty", p, np.zeros((0, 4), np.float32)))
+    pos = np.ones((64, 4), np.float32)                        # points on cell faces
+    pos[:, :3] = (rng.integers(0, 40, (64, 3)) * p.cell_size).astype(np.float32)
+    out.append(("faces", p, pos))
+    return out
+
+
+def test_grid_build_bit_exact(gpu, cwa, oracle):
+    import torch
+
+    for name, p, pos in _grid_cases(cwa):
+        _say(f"grid case {name} n={pos.shape[0]}")
+        g = _gpu_grid(cwa, torch, gpu, p, pos)
+        r = oracle.grid_build(p, pos)
+        for k in ("key", "count", "offset", "list"):
+            assert np.array_equal(g[k], r[k]), (name, k)
+
+
+def test_grid_build_bit_exact_1m(gpu, cwa, oracle):
+    import torch
+    from coupledwateranimation_amd import configs
+
+    s = configs.dam_break_large(steps=1)
+    _say("grid 1M")
+    g = _gpu_grid(cwa, torch, gpu, s.params, s.pos)
+    r = oracle.grid_build(s.params, s.pos)
+    for k in ("key", "count", "offset", "list"):
+        assert np.array_equal(g[k], r[k]), k
+
+
+# ----------------------------------------------------------------- wave
+def _wave_params(cwa, wtype=1.0):
+    p = cwa.reference_params()
+    p.lambda_x, p.lambda_z, p.atten, p.beta, p.wave_type = 0.11, 0.07, 0.97, 0.003, wtype
+    return p
+
+
+@pytest.mark.parametrize("wtype,W,H", [(1.0, 96, 64), (0.5, 130, 67), (1.0, 1, 1),
+                                       (1.0, 2048, 2048)])
+def test_wave_only_bit_exact(gpu, cwa, oracle, wtype, W, H):
+    _say(f"wave {W}x{H} type={wtype}")
+    p = _wave_params(cwa, wtype)
+    rng = np.random.default_rng(W + H)
+    h0 = rng.normal(size=(H, W)).astype(np.float32) * 0.01
+    h1 = rng.normal(size=(H, W)).astype(np.float32) * 0.01
+    steps = 12 if W < 1000 else 3
+    with cwa.Scene(p, np.zeros((0, 4), np.float32), wave_size=(W, H), height0=h0,
+                   height1=h1) as s:
+        a, b = h0.copy(), h1.copy()
+        for _ in range(steps):
+            s.step(1e-3)   # no particles: the step is the wave EVOLVE + PingPong
+            a, b = oracle.wave_evolve(p, a, b), a
+        st = s.readback()
+    assert np.array_equal(st["height0"], a) and np.array_equal(st["height1"], b)
+
+
+def test_wave_compute_and_init_modes(gpu, cwa, oracle):
+    p = _wave_params(cwa)
+    W, H = 80, 72
+    with cwa.Scene(p, np.zeros((0, 4), np.float32), wave_size=(W, H)) as s:
+        for t in (1.0, 0.0, 0.5):
+            q = p.copy()
+            q.wave_type = t
+            s.set_params(q)
+            s.wave_reinit()
+            st = s.readback()
+            ref = oracle.init_wave(t, W, H)
+            assert np.array_equal(st["height0"], ref) and np.array_equal(st["height1"], ref)
+        s.wave_compute(1e-3)
+        st = s.readback()
+        q = p.copy()
+        q.wave_type = 0.5
+        ref = oracle.init_wave(0.5, W, H)
+        assert np.array_equal(st["height0"], oracle.wave_evolve(q, ref, ref))
+        # ReinitFromTexture: texel (i, j) <- tex[j, 2 i]; newest only
+        tex = np.random.default_rng(1).normal(size=(H, 2 * W)).astype(np.float32)
+        prev = st["height0"].copy()
+        s.wave_reinit_from_texture(tex)
+        st = s.readback()
+        assert np.array_equal(st["height0"], tex[:, ::2]) and np.array_equal(st["height1"], prev)
+
+
+# ----------------------------------------------------------------- particles
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def _gpu_run(cwa, spec, steps, graph=False):
+    s = spec.make_scene()
+    s.step(spec.dt, steps, graph=graph)
+    st = s.readback()
+    g = s.grid_host()
+    s.close()
+    return st, g
+
+
+def _oracle_run(oracle, spec, steps, double=False, threads=8):
+    o = oracle.OracleScene(spec.params, spec.pos, spec.vel, None, spec.wave_size, spec.height0,
+                           spec.height1, double=double, threads=threads)
+    o.step(spec.dt, steps)
+    return o, o.state()
+
+
+def _specs(cwa):
+    from coupledwateranimation_amd import configs
+
+    return {"dam_break_4k": configs.dam_break_small(), "reference": configs.reference_scene()}
+
+
+@pytest.mark.parametrize("name", ["dam_break_4k", "reference"])
+def test_one_step_parity(gpu, cwa, oracle, name):
+    """One coupled step: grid arrays bit-exact; rho, p relative 2e-6;
+    force relative to its max 2e-5; positions 1e-6 of the box; velocities
+    relative 2e-5; heights relative 1e-6."""
+    spec = _specs(cwa)[name]
+    _say(f"one step {name}")
+    st, g = _gpu_run(cwa, spec, 1)
+    o, ref = _oracle_run(oracle, spec, 1)
+    og = o.grid(spec.params)
+    for k in ("key", "count", "offset", "list"):
+        assert np.array_equal(g[k], og[k]), k
+    assert _rel(st["rho"], ref["rho"]) < 2e-6
+    assert _rel(st["prs"], ref["prs"]) < 2e-6
+    assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
+    assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
+    assert _rel(st["vel"][:, :3], ref["vel"][:, :3]) < 2e-5
+    assert _rel(st["height0"], ref["height0"]) < 1e-6 or np.abs(ref["height0"]).max() == 0
+    assert np.allclose(g["colflux"], og["colflux"], rtol=1e-5,
+                       atol=1e-6 * max(np.abs(og["colflux"]).max(), 1e-30))
+
+
+def test_n_step_parity_vs_fp64(gpu, cwa, oracle):
+    """50 coupled steps of the 4 096-particle dam break.  Error of the GPU
+    against the FP64 oracle, per field, must stay within 20x the FP32
+    oracle's own error plus a floor of 1e-6 (relative to the field max)."""
+    spec = _specs(cwa)["dam_break_4k"]
+    steps = 50
+    st, _ = _gpu_run(cwa, spec, steps)
+    _, f32 = _oracle_run(oracle, spec, steps)
+    _, f64 = _oracle_run(oracle, spec, steps, double=True)
+    for k, sl in (("pos", np.s_[:, :3]), ("vel", np.s_[:, :3]), ("rho", np.s_[:]),
+                  ("prs", np.s_[:]), ("height0", np.s_[:])):
+        e_gpu = _rel(st[k][sl], f64[k][sl])
+        e_f32 = _rel(f32[k][sl], f64[k][sl])
+        _say(f"{k}: gpu-vs-fp64 {e_gpu:.3e}  fp32oracle-vs-fp64 {e_f32:.3e}")
+        assert e_gpu <= 20 * e_f32 + 1e-6, k
+
+
+def test_branch_scenes(gpu, cwa, oracle):
+    """Tiny scenes that drive every branch: boundary faces, foam, surface
+    clamp, coupling band with flux, wake ceiling mode, coincident pair."""
+    from coupledwateranimation_amd import configs
+
+    base = configs.dam_break_small()
+    p = base.params.copy()
+    p.torque_coeff = 0.25
+    W = H = 16
+    jj, ii = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    h0 = (0.02 + 0.003 * ii - 0.002 * jj).astype(np.float32)
+    pos = np.array([[0.0005, 0.5, 0.5, 1], [0.9995, 0.3, 0.5, 1], [0.5, 0.0005, 0.5, 1],
+                    [0.5, 0.9995, 0.5, 1], [0.5, 0.5, 0.0005, 1], [0.5, 0.5, 0.9995, 1],
+                    [0.3, 0.4, 0.3, 1], [0.7, 0.03, 0.2, 1], [0.2, 0.01, 0.8, 1],
+                    [0.0, 0.0, 0.0, 1], [0.0, 0.0, 0.0, 1], [0.6, 0.035, 0.6, 1],
+                    [0.61, 0.04, 0.6, 1]], np.float32)
+    vel = np.zeros_like(pos)
+    vel[:6, :3] = [[-2, 0, 0], [2, 0, 0], [0, -2, 0], [0, 2, 0], [0, 0, -2], [0, 0, 2]]
+    vel[6] = [0, 0, 30, 0]          # foam
+    vel[7] = [0.1, -0.5, 0, 0]      # in band, flux
+    vel[11] = [0, -0.3, 0.1, 0]
+    force = np.zeros_like(pos)
+    force[:, :3] = [1.0, -2.0, 0.5]
+    force[6, 3] = 2.0
+    for wtype in (1.0, 0.5):
+        q = p.copy()
+        q.wave_type = wtype
+        s = cwa.Scene(q, pos, vel, force, (W, H), h0, h0)
+        s.step(1e-3, 3)
+        st = s.readback()
+        s.close()
+        o = oracle.OracleScene(q, pos, vel, force, (W, H), h0, h0)
+        o.step(1e-3, 3)
+        ref = o.state()
+        for k in ("pos", "vel", "force"):
+            np.testing.assert_allclose(st[k], ref[k], rtol=2e-5, atol=2e-6 * np.abs(ref[k]).max())
+        np.testing.assert_allclose(st["rho"], ref["rho"], rtol=2e-6)
+        np.testing.assert_allclose(st["height0"], ref["height0"], rtol=1e-5,
+                                   atol=1e-6 * np.abs(ref["height0"]).max())
+        assert np.isfinite(st["pos"]).all()
+
+
+def test_graph_equals_eager_and_deterministic(gpu, cwa):
+    """hipGraph replay and eager launches give bit-identical states; two
+    eager runs are bit-identical (no float atomics anywhere)."""
+    spec = _specs(cwa)["dam_break_4k"]
+    a, _ = _gpu_run(cwa, spec, 31)
+    b, _ = _gpu_run(cwa, spec, 31)
+    c, _ = _gpu_run(cwa, spec, 31, graph=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+
+
+def test_empty_and_edge_scenes(gpu, cwa):
+    p = cwa.reference_params()
+    with cwa.Scene(p, np.zeros((0, 4), np.float32), wave_size=(1, 1)) as s:
+        s.step(5e-5, 3)
+        assert s.readback()["pos"].shape == (0, 4)
+    pos = np.full((500, 4), 0.25, np.float32)      # all particles on one point
+    with cwa.Scene(p, pos, wave_size=(4, 4)) as s:
+        s.step(5e-5, 2)
+        st = s.readback()
+        assert np.isfinite(st["pos"]).all()
+    q = p.copy()
+    q.check_finite = 1
+    bad = np.full((4, 4), np.nan, np.float32)
+    with cwa.Scene(q, np.zeros((0, 4), np.float32), wave_size=(4, 4), height0=bad) as s:
+        with pytest.raises(cwa.CwaError) as e:
+            s.step(5e-5, 1)
+        assert e.value.code == -5
+
+
+def test_large_scene_step(gpu, cwa, oracle):
+    """1 048 576 particles + 2048^2 heightfield: one step against the oracle
+    (16 threads), then 10 more steps checked through size-independent
+    properties: ids stay a permutation, slots stay cell-sorted, all finite,
+    particles inside the box."""
+    from coupledwateranimation_amd import configs
+
+    spec = configs.dam_break_large(steps=1)
+    _say("large: gpu step")
+    s = spec.make_scene()
+    s.step(spec.dt, 1)
+    st = s.readback()
+    g = s.grid_host()
+    _say("large: oracle step")
+    o, ref = _oracle_run(oracle, spec, 1, threads=16)
+    og = o.grid(spec.params)
+    for k in ("key", "count", "offset", "list"):
+        assert np.array_equal(g[k], og[k]), k
+    assert _rel(st["rho"], ref["rho"]) < 2e-6
+    assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
+    assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
+    assert _rel(st["height0"], ref["height0"]) < 1e-6
+    _say("large: 10 more steps")
+    s.step(spec.dt, 10)
+    ids = s.slot_ids()
+    assert np.array_equal(np.sort(ids), np.arange(spec.n))
+    st = s.readback()
+    g = s.grid_host()
+    assert np.isfinite(st["pos"]).all() and np.isfinite(st["height0"]).all()
+    assert np.all(np.diff(g["key"][g["list"]]) >= 0)
+    lo, hi = np.array(spec.params.lower), np.array(spec.params.upper)
+    assert np.all(st["pos"][:, :3] >= lo) and np.all(st["pos"][:, :3] <= hi)
+    s.close()
