@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", choices=["large", "small"], default="large")
     ap.add_argument("--eager", action="store_true", help="launch kernels without hipGraph")
-    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps of the per-kernel event pass (0: the same K as timed)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="0 disables the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
     return ap.parse_args()
@@ -90,12 +91,21 @@ def main() -> int:
     ms_step = 1e3 * t / args.steps
     print(f"bench: timed {args.steps} steps in {t:.3f} s ({ms_step:.4f} ms/step)", flush=True)
 
-    # per-kernel device time, HIP events on the scene stream (same workload,
-    # continuing the same trajectory)
-    prof = scene.step_profiled(spec.dt, args.profile_steps)
-    kernel_ms = {k: v[0] for k, v in prof.items()}
     st = scene.readback()
     finite = bool(np.isfinite(st["pos"]).all() and np.isfinite(st["height0"]).all())
+    state_stats = {"rho_min": float(st["rho"].min()), "rho_max": float(st["rho"].max()),
+                   "rho_mean": float(st["rho"].mean()),
+                   "vmax": float(np.abs(st["vel"][:, :3]).max()),
+                   "hmax": float(np.abs(st["height0"]).max())}
+
+    # Per-kernel device time from HIP events on the scene's stream, over the
+    # same trajectory: back to the initial state, the same warm-up, then the
+    # same K steps launched eagerly with an event pair around each kernel.
+    scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+    scene.step(spec.dt, args.warmup)
+    prof = scene.step_profiled(spec.dt, min(args.steps, args.profile_steps)
+                               if args.profile_steps > 0 else args.steps)
+    kernel_ms = {k: v[0] for k, v in prof.items()}
     scene.close()
 
     units = {"force_integrate": n, "density": n, "hash": n, "reorder": n}
@@ -162,6 +172,7 @@ def main() -> int:
                      "traffic": traffic, "alg_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "finite": finite,
+        "state_after": state_stats,
     }
     print(json.dumps(out), flush=True)
     return 0

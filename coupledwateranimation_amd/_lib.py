@@ -154,6 +154,14 @@ def lib() -> C.CDLL:
             raise RuntimeError(
                 f"HIP library not built: {LIB_PATH} is missing "
                 "(run `python -m coupledwateranimation_amd.build`)")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Loading torch first makes libcwa bind to that
+        # one, so the process has a single HIP runtime; loading libcwa first
+        # would map a second runtime and torch would then see no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

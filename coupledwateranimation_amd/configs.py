@@ -151,7 +151,7 @@ def dam_break_small(seed: int = 0, steps: int = 200) -> SceneSpec:
 
 
 def dam_break_large(seed: int = 1, steps: int = 1000, n: int = 1 << 20,
-                    wave: int = 2048) -> SceneSpec:
+                    wave: int = 2048, stiffness: float = 1000.0, dt: float = 1e-4) -> SceneSpec:
     """Plain coupled 3D step at scale: 1 048 576 particles + 2048^2 heightfield.
 
     Box [0,2]x[0,1]x[0,1]; block = the x-prefix slab holding 40% of the box
@@ -160,14 +160,18 @@ def dam_break_large(seed: int = 1, steps: int = 1000, n: int = 1 << 20,
     +-0.1 s; h = 1.95 s so that (4/3) pi (h/s)^3 ~ 31 neighbours; grid cell = h.
     Heightfield: 8 Gaussian bumps, amplitude U[0.01, 0.05], sigma U[5, 40]
     texels, bump seed = seed + 0x5EED; both history levels equal.
-    c = 1 and dt = 2e-4 (dt = 1e-3 would break the wave CFL limit on a
-    2048-texel unit axis); attenuation 0.999."""
+    Stiffness k = 1000 (sound speed ~32 m/s, ~10x the dam's fall speed:
+    the weakly compressible regime, so the column stays near rest density
+    and keeps ~30 neighbours per particle; the small workload's k = 3 would
+    compress a metre-high column about 4x).  c = 1 and dt = 1e-4 (dt = 1e-3
+    would break the wave CFL limit on a 2048-texel unit axis; 1e-4 keeps
+    the SPH acoustic CFL number at ~0.18); attenuation 0.999."""
     s = (0.8 / n) ** (1.0 / 3.0)
     h = 1.95 * s
-    dt = 2e-4
     W = H = wave
     pos = lattice(n, (0.0, 0.0, 0.0), (0.8, 1.0, 1.0), s, 0.5, 0.1, (2, 1, 0), seed)
     p = _textbook_params(h, s, (0.0, 0.0, 0.0), (2.0, 1.0, 1.0), W, H, dt, 1.0, 0.999)
+    p.gas_const = stiffness
     h0 = bumps(W, H, 8, (0.01, 0.05), (5.0, 40.0), seed + 0x5EED)
     return SceneSpec(f"dam_break_{n}_{wave}", p, pos, np.zeros_like(pos), (W, H), h0,
                      h0.copy(), dt, steps, {"spacing": s})

@@ -365,6 +365,8 @@ void CoupledScene::Reinit(const cwa_init& init) {
     hip_check(launch_fill_i32(grid.count(), grid.num_cells(), 0, st_), "count zero");
     hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
     colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(colflux.get()), (int)colflux.size(), 0, st_),
+              "colflux zero");
     wave.Allocate(init.wave_w, init.wave_h);
     wave.Upload(init.height0, init.height1, st_);
     build_devparams();
@@ -389,6 +391,8 @@ void CoupledScene::SetParams(const cwa_params& p) {
         hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
         colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
     }
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(colflux.get()), (int)colflux.size(), 0, st_),
+              "colflux zero");
     build_devparams();
 }
 

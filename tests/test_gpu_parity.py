@@ -204,7 +204,9 @@ def _specs(cwa):
 
 @pytest.mark.parametrize("name", ["dam_break_4k", "reference"])
 def test_one_step_parity(gpu, cwa, oracle, name):
-    """One coupled step: grid arrays bit-exact; rho, p relative 2e-6;
+    """One coupled step: grid arrays bit-exact; rho relative 2e-6; pressure
+    p = k (rho - rho0) + H rho within 2e-6 (k max rho + max |p|) (the
+    subtraction turns rho's rounding into a larger relative error of p);
     force relative to its max 2e-5; positions 1e-6 of the box; velocities
     relative 2e-5; heights relative 1e-6."""
     spec = _specs(cwa)[name]
@@ -215,7 +217,8 @@ def test_one_step_parity(gpu, cwa, oracle, name):
     for k in ("key", "count", "offset", "list"):
         assert np.array_equal(g[k], og[k]), k
     assert _rel(st["rho"], ref["rho"]) < 2e-6
-    assert _rel(st["prs"], ref["prs"]) < 2e-6
+    pbar = 2e-6 * (spec.params.gas_const * ref["rho"].max() + np.abs(ref["prs"]).max())
+    assert np.abs(st["prs"] - ref["prs"]).max() <= pbar
     assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
     assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
     assert _rel(st["vel"][:, :3], ref["vel"][:, :3]) < 2e-5
