@@ -49,12 +49,10 @@ struct DevParams {
     int num_cells;
 };
 
-// ---------------------------------------------------------------------------
-// Arithmetic that must round exactly like the oracle: no contraction.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float dist2_exact(float dx, float dy, float dz) {
-#pragma clang fp contract(off)
-    return dx * dx + dy * dy + dz * dz;
+// Squared distance as the fused chain fma(dz, dz, fma(dy, dy, dx*dx)); the
+// oracle evaluates the same chain, so both see the same neighbour set.
+__device__ __forceinline__ float dist2_fma(float dx, float dy, float dz) {
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 // Cell of a point: clamp(floor((p - gmin)/cell), 0, dims-1); the division

@@ -174,18 +174,39 @@ hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st) 
 // atomic return value is a provisional rank; the stable order is restored
 // by k_stable_reorder.
 // ---------------------------------------------------------------------------
-__global__ void k_hash_count(const float4* __restrict__ pos, int n, GridDesc g,
-                             int* __restrict__ key, int* __restrict__ trank,
-                             int* __restrict__ count) {
+__global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ pos, int n,
+                                                    GridDesc g, int* __restrict__ key,
+                                                    int* __restrict__ trank,
+                                                    int* __restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
-    const int ix = cell_axis(p.x, g.gmin[0], g.cell, g.dims[0]);
-    const int iy = cell_axis(p.y, g.gmin[1], g.cell, g.dims[1]);
-    const int iz = cell_axis(p.z, g.gmin[2], g.cell, g.dims[2]);
-    const int c = lin_cell(g.dims, ix, iy, iz);
-    key[i] = c;
-    trank[i] = atomicAdd(&count[c], 1);
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    int c = -1;
+    if (valid) {
+        const float4 p = pos[i];
+        const int ix = cell_axis(p.x, g.gmin[0], g.cell, g.dims[0]);
+        const int iy = cell_axis(p.y, g.gmin[1], g.cell, g.dims[1]);
+        const int iz = cell_axis(p.z, g.gmin[2], g.cell, g.dims[2]);
+        c = lin_cell(g.dims, ix, iy, iz);
+        key[i] = c;
+    }
+    // Wave-aggregated atomics: the slots of a wave are nearly sorted by cell,
+    // so lanes sharing a cell take one atomicAdd per group (leader = lowest
+    // lane) and number themselves by lane order inside the group.
+    unsigned long long pending = __ballot(valid);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int rank = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lk = __shfl(c, leader, 64);
+        const unsigned long long grp = __ballot(valid && c == lk);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[lk], __popcll(grp));
+        base = __shfl(base, leader, 64);
+        if (valid && c == lk) rank = base + __popcll(grp & lt);
+        pending &= ~grp;
+    }
+    if (valid) trank[i] = rank;
 }
 
 // InsertParticle (ugrid_particles_cs.glsl:110-117) into a provisional list.
@@ -225,33 +246,43 @@ __global__ void k_stable_reorder(const int* __restrict__ key, const int* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour iteration: 9 columns (dx outer, dz inner), each the contiguous
-// sorted range of cells iy-1..iy+1.  Same order as the oracle.
+// Neighbour columns: candidate column c = 0..8 is (dx, dz) = (c/3 - 1,
+// c%3 - 1) (dx outer, dz inner, the oracle's order); its candidates are the
+// contiguous sorted range of cells iy-1..iy+1 of that column.
 // ---------------------------------------------------------------------------
-template <class F>
-__device__ __forceinline__ void for_each_candidate(const DevParams& P,
-                                                   const int* __restrict__ offset, int ix, int iy,
-                                                   int iz, F&& f) {
-    const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.dims[1] - 1);
-    for (int dx = -1; dx <= 1; ++dx) {
-        const int cx = ix + dx;
-        if (cx < 0 || cx >= P.dims[0]) continue;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int cz = iz + dz;
-            if (cz < 0 || cz >= P.dims[2]) continue;
-            const int b = offset[lin_cell(P.dims, cx, ylo, cz)];
-            const int e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
-            for (int j = b; j < e; ++j) f(j);
-        }
+__device__ __forceinline__ void column_range(const DevParams& P, const int* __restrict__ offset,
+                                             int ix, int iy, int iz, int c, int& b, int& e) {
+    const int cx = ix + c / 3 - 1, cz = iz + c % 3 - 1;
+    if (cx < 0 || cx >= P.dims[0] || cz < 0 || cz >= P.dims[2]) {
+        b = e = 0;
+        return;
     }
+    const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.dims[1] - 1);
+    b = offset[lin_cell(P.dims, cx, ylo, cz)];
+    e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
 }
 
 __device__ __forceinline__ bool coupled(const DevParams& P, float y, float H) {
     return !(P.band > 0.0f) || fabsf(y - H) < P.band;
 }
 
+// One density candidate: w = max(T - r2, 0) with T the exact threshold
+// (r < h  <=>  r2 < T), so w > 0 exactly for neighbours; accumulates w^3 and
+// sets mask bit k for a neighbour.
+__device__ __forceinline__ void density_candidate(const float4 pi, const float4 pj, float T,
+                                                  float& acc, uint32_t& word, int k) {
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float r2 = dist2_fma(dx, dy, dz);
+    const float w = fmaxf(T - r2, 0.0f);
+    acc = fmaf(w * w, w, acc);
+    word |= min(__float_as_uint(w), 1u) << k;
+}
+
 // ---------------------------------------------------------------------------
 // Density / pressure (rho_pres_comp.glsl:49-81) + particle flux term.
+// Also records the exact neighbour set of every particle: per candidate
+// column a 64-bit mask over its first 64 candidates (nbmask[c*n + i];
+// candidates past 64 are re-tested by the force pass).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
@@ -259,22 +290,39 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
                                                  const float* __restrict__ H0,
                                                  float* __restrict__ rho_w,
                                                  float2* __restrict__ pw,
-                                                 float* __restrict__ flux) {
+                                                 float* __restrict__ flux,
+                                                 uint2* __restrict__ nbmask) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
     const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+    const float T = P.r2_lt;
     float acc = 0.0f;
-    for_each_candidate(P, offset, ix, iy, iz, [&](int j) {
-        const float4 pj = pos[j];
-        const float r2 = dist2_exact(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-        if (r2 < P.r2_lt) {
-            const float q = P.h2 - r2;
-            acc += q * q * q;
+#pragma unroll 1
+    for (int c = 0; c < 9; ++c) {
+        int b, e;
+        column_range(P, offset, ix, iy, iz, c, b, e);
+        const float4* __restrict__ q = pos + b;
+        const int len = e - b;
+        uint32_t lo = 0, hi = 0;
+        const int l1 = min(len, 32);
+        int k = 0;
+        for (; k + 1 < l1; k += 2) {
+            const float4 a0 = q[k], a1 = q[k + 1];
+            density_candidate(pi, a0, T, acc, lo, k);
+            density_candidate(pi, a1, T, acc, lo, k + 1);
         }
-    });
+        if (k < l1) density_candidate(pi, q[k], T, acc, lo, k);
+        if (len > 32) {
+            const int l2 = min(len, 64);
+            for (k = 32; k < l2; ++k) density_candidate(pi, q[k], T, acc, hi, k - 32);
+            uint32_t none = 0;
+            for (k = 64; k < len; ++k) density_candidate(pi, q[k], T, acc, none, 0);
+        }
+        nbmask[(size_t)c * n + i] = make_uint2(lo, hi);
+    }
     float rho = P.m_poly6 * acc;
     float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
     const float u = (pi.x - P.org[0]) * P.uvs[0];
@@ -292,26 +340,57 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
     if (P.band > 0.0f) {
         const float d = fabsf(pi.y - height);
         if (d < P.band) {
-            const float q = d / P.band;
-            const float w = (1.0f - q * q) * (1.0f - q * q);
+            const float qq = d / P.band;
+            const float w = (1.0f - qq * qq) * (1.0f - qq * qq);
             fl = P.m_rho0 * vel[i].y * w;
         }
     }
     flux[i] = fl;
 }
 
+// Pressure + viscosity contribution of neighbour j (force_comp.glsl:85-86).
+struct PairAcc {
+    float pfx, pfy, pfz, vfx, vfy, vfz;
+};
+
+__device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4 vi, float p_i,
+                                           float m_spiky_half, float m_lap,
+                                           const float4* __restrict__ pos,
+                                           const float4* __restrict__ vel,
+                                           const float2* __restrict__ pw, int j, PairAcc& a) {
+    const float4 pj = pos[j];
+    const float2 q = pw[j];
+    const float4 vj = vel[j];
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float r2 = dist2_fma(dx, dy, dz);
+    // an exactly coincident pair (possible after boundary clamping) has no
+    // pressure direction: rinv = 0 drops that term
+    const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float r = r2 * rinv;
+    const float hr = P.h - r;
+    const float s = m_spiky_half * (p_i + q.x) * q.y * hr * hr * rinv;
+    a.pfx -= s * dx;
+    a.pfy -= s * dy;
+    a.pfz -= s * dz;
+    const float t = m_lap * q.y * hr;
+    a.vfx += t * (vj.x - vi.x);
+    a.vfy += t * (vj.y - vi.y);
+    a.vfz += t * (vj.z - vi.z);
+}
+
 // ---------------------------------------------------------------------------
 // Forces (force_comp.glsl:62-139) fused with symplectic Euler + boundary
-// (integrate_comp.glsl:56-92, 135-178).  Reads the sorted copy B, writes
-// the state A in the same (sorted) slot order.
+// (integrate_comp.glsl:56-92, 135-178).  Visits only the neighbours the
+// density pass recorded (bit order = candidate order), reads the sorted
+// copy B and writes the state A in the same slot order.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_force_integrate(
     DevParams P, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
     int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
-    const float* __restrict__ H0, float4* __restrict__ pos_o, float4* __restrict__ vel_o,
-    float4* __restrict__ force_o, int* __restrict__ id_o, float* __restrict__ rho_o,
-    float* __restrict__ prs_o) {
+    const uint2* __restrict__ nbmask, const float* __restrict__ H0, float4* __restrict__ pos_o,
+    float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
+    float* __restrict__ rho_o, float* __restrict__ prs_o) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
@@ -322,29 +401,31 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float m_spiky_half = P.m * P.spiky * 0.5f;
     const float m_lap = P.m * P.lap;
-    float pfx = 0.f, pfy = 0.f, pfz = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
-    for_each_candidate(P, offset, ix, iy, iz, [&](int j) {
-        const float4 pj = pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2_exact(dx, dy, dz);
-        if (r2 < P.r2_lt && j != i) {
-            // an exactly coincident pair (possible after boundary clamping)
-            // has no pressure direction: rinv = 0 drops that term
-            const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-            const float r = r2 * rinv;
-            const float hr = P.h - r;
-            const float2 q = pw[j];
-            const float s = m_spiky_half * (p_i + q.x) * q.y * hr * hr * rinv;
-            pfx -= s * dx;
-            pfy -= s * dy;
-            pfz -= s * dz;
-            const float4 vj = vel[j];
-            const float t = m_lap * q.y * hr;
-            vfx += t * (vj.x - vi.x);
-            vfy += t * (vj.y - vi.y);
-            vfz += t * (vj.z - vi.z);
+    PairAcc a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int c = 0; c < 9; ++c) {
+        int b, e;
+        column_range(P, offset, ix, iy, iz, c, b, e);
+        const uint2 mk = nbmask[(size_t)c * n + i];
+        uint32_t lo = mk.x, hi = mk.y;
+        while (lo) {
+            const int j = b + __builtin_ctz(lo);
+            lo &= lo - 1u;
+            if (j != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
         }
-    });
+        while (hi) {
+            const int j = b + 32 + __builtin_ctz(hi);
+            hi &= hi - 1u;
+            if (j != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
+        }
+        for (int j = b + 64; j < e; ++j) {  // columns longer than 64 candidates
+            const float4 pj = pos[j];
+            const float r2 = dist2_fma(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            if (r2 < P.r2_lt && j != i)
+                force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
+        }
+    }
+    float vfx = a.vfx, vfy = a.vfy, vfz = a.vfz;
     float4 fp = force[i];
     if (pi.y > P.crest) {
         fp.x /= P.brk_mass;
@@ -382,9 +463,9 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     }
     float rho = rho_w[i];
     float prs = p_i;
-    float Fx = pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
-    float Fy = pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
-    float Fz = pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
+    float Fx = a.pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
+    float Fy = a.pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
+    float Fz = a.pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
     float Fw = fp.w;
 
     // integrate_comp.glsl:56-92
@@ -394,7 +475,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     nvx *= damp;
     nvy *= damp;
     nvz *= damp;
-    if (sqrtf(dist2_exact(nvx, nvy, nvz)) > P.foam_speed) {
+    if (sqrtf(dist2_fma(nvx, nvy, nvz)) > P.foam_speed) {
         Fx *= P.foam_f;
         Fy *= P.foam_f;
         Fz *= P.foam_f;
@@ -481,6 +562,48 @@ __global__ void __launch_bounds__(256) k_wave(DevParams P, const float* __restri
     out[c] = o;
 }
 
+// The same EVOLVE, four texels per thread along u (W % 4 == 0): float4
+// loads of the centre, north and south rows and of h^(n-1), scalar loads of
+// the west / east neighbours of the quad.  Per texel the arithmetic is that
+// of k_wave, so both give identical results.
+__device__ __forceinline__ float wave_texel(const DevParams& P, float C0, float C1, float nn,
+                                            float ss, float ee, float ww, int i, int j,
+                                            const float* __restrict__ colflux, float dt) {
+#pragma clang fp contract(off)
+    float o = P.wc0 * C0 + P.lx * (ee + ww) + P.lz * (nn + ss) - P.wc1 * C1;
+    o *= P.atten;
+    if (colflux) {
+        const float gx = P.ax * (float)i + P.bx, gz = P.az * (float)j + P.bz;
+        const float S = bilerp(colflux, P.dims[0], P.dims[2], gx, gz);
+        o += P.kappa * dt * (S * P.inv_area);
+    }
+    if (P.wake && o > P.wake_thr && i == P.W / 2) o += P.wake_amt;
+    return o;
+}
+
+__global__ void __launch_bounds__(256) k_wave4(DevParams P, const float* __restrict__ h0,
+                                               const float* __restrict__ h1,
+                                               float* __restrict__ out,
+                                               const float* __restrict__ colflux, float dt) {
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int W = P.W, H = P.H;
+    if (i0 >= W || j >= H) return;
+    const size_t row = (size_t)j * W;
+    const float4 c0 = *reinterpret_cast<const float4*>(h0 + row + i0);
+    const float4 c1 = *reinterpret_cast<const float4*>(h1 + row + i0);
+    const float4 nn = *reinterpret_cast<const float4*>(h0 + (size_t)min(j + 1, H - 1) * W + i0);
+    const float4 ss = *reinterpret_cast<const float4*>(h0 + (size_t)max(j - 1, 0) * W + i0);
+    const float west = h0[row + max(i0 - 1, 0)];
+    const float east = h0[row + min(i0 + 4, W - 1)];
+    float4 o;
+    o.x = wave_texel(P, c0.x, c1.x, nn.x, ss.x, c0.y, west, i0, j, colflux, dt);
+    o.y = wave_texel(P, c0.y, c1.y, nn.y, ss.y, c0.z, c0.x, i0 + 1, j, colflux, dt);
+    o.z = wave_texel(P, c0.z, c1.z, nn.z, ss.z, c0.w, c0.y, i0 + 2, j, colflux, dt);
+    o.w = wave_texel(P, c0.w, c1.w, nn.w, ss.w, east, c0.z, i0 + 3, j, colflux, dt);
+    *reinterpret_cast<float4*>(out + row + i0) = o;
+}
+
 // InitWave (wave_comp.glsl:82-140): peak * smoothstep(5, 0, d), d = distance
 // to the nearest of 2 or 3 centres.
 __global__ void k_init_wave(float* __restrict__ out, int W, int H, int c0x, int c0y, int c1x,
@@ -546,22 +669,22 @@ hipError_t launch_stable_reorder(const int* key, const int* offset, const int* t
 }
 
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
-                          const float* H0, float* rho_w, float2* pw, float* flux,
+                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
                           hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_density<<<blocks_for(n, 256), 256, 0, st>>>(P, B.pos, B.vel, offset, n, H0, rho_w, pw,
-                                                  flux);
+                                                  flux, nbmask);
     return hipGetLastError();
 }
 
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const float* H0, ParticleArrays& A, float* rho, float* prs,
-                                  hipStream_t st) {
+                                  const uint2* nbmask, const float* H0, ParticleArrays& A,
+                                  float* rho, float* prs, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P, dt, B.pos, B.vel, B.force, B.id,
-                                                          offset, n, rho_w, pw, H0, A.pos, A.vel,
-                                                          A.force, A.id, rho, prs);
+                                                          offset, n, rho_w, pw, nbmask, H0,
+                                                          A.pos, A.vel, A.force, A.id, rho, prs);
     return hipGetLastError();
 }
 
@@ -578,8 +701,13 @@ hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, flo
                        const float* colflux, float dt, hipStream_t st) {
     if (P.W <= 0 || P.H <= 0) return hipSuccess;
     dim3 blk(64, 4);
-    dim3 grd((P.W + 63) / 64, (P.H + 3) / 4);
-    k_wave<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+    if (P.W % 4 == 0) {
+        dim3 grd((P.W / 4 + 63) / 64, (P.H + 3) / 4);
+        k_wave4<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+    } else {
+        dim3 grd((P.W + 63) / 64, (P.H + 3) / 4);
+        k_wave<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+    }
     return hipGetLastError();
 }
 

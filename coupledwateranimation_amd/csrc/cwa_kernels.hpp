@@ -33,13 +33,14 @@ hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offse
 hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
                                  int* list, const ParticleArrays* in, ParticleArrays* out,
                                  hipStream_t st);
+// nbmask: 9*n uint2, written by the density pass, read by the force pass
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
-                          const float* H0, float* rho_w, float2* pw, float* flux,
+                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
                           hipStream_t st);
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const float* H0, ParticleArrays& A, float* rho, float* prs,
-                                  hipStream_t st);
+                                  const uint2* nbmask, const float* H0, ParticleArrays& A,
+                                  float* rho, float* prs, hipStream_t st);
 hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
                               float* colflux, hipStream_t st);
 hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,

@@ -170,9 +170,11 @@ class SphParticles3D {
     float* rho_w() const { return rho_w_.get(); }
     float2* pw() const { return pw_.get(); }
     float* flux() const { return flux_.get(); }
+    uint2* nbmask() const { return nbmask_.get(); }
 
    private:
     int n_ = 0;
+    DeviceBuffer<uint2> nbmask_;  // 9 x n neighbour bitmasks of the last step
     DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
     DeviceBuffer<int> id_[2];
     DeviceBuffer<float> rho_, prs_, rho_w_, flux_;

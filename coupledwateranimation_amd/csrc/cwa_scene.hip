@@ -178,6 +178,7 @@ void SphParticles3D::Allocate(int n) {
     rho_w_.resize(m);
     flux_.resize(m);
     pw_.resize(m);
+    nbmask_.resize(9 * m);
     A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
     B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
 }
@@ -408,13 +409,13 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2]) {
                        ev ? &ev[0][0] : nullptr);
         mark(K_DENSITY, 0);
         hip_check(launch_density(dp_, particles.B, grid.offset(), n, H0, particles.rho_w(),
-                                 particles.pw(), particles.flux(), st_),
+                                 particles.pw(), particles.flux(), particles.nbmask(), st_),
                   "density");
         mark(K_DENSITY, 1);
         mark(K_FORCE, 0);
         hip_check(launch_force_integrate(dp_, dt, particles.B, grid.offset(), n, particles.rho_w(),
-                                         particles.pw(), H0, particles.A, particles.rho(),
-                                         particles.prs(), st_),
+                                         particles.pw(), particles.nbmask(), H0, particles.A,
+                                         particles.rho(), particles.prs(), st_),
                   "force/integrate");
         mark(K_FORCE, 1);
         if (couple) {

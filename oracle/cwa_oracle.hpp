@@ -314,11 +314,14 @@ struct Arrays {
     std::vector<int32_t> id;         // n
 };
 
-// length(delta) < smoothing_length with length = sqrt(dx^2+dy^2+dz^2)
-// (rho_pres_comp.glsl:62-64).
+// length(delta) < smoothing_length (rho_pres_comp.glsl:62-64).  GLSL leaves
+// the rounding of length() to the implementation; here the squared length
+// is the fused chain fma(dz, dz, fma(dy, dy, dx*dx)) (each step rounded
+// once, as IEEE fma defines) and length = sqrt of it, so the neighbour set
+// is fixed bit for bit.
 template <class R>
 static inline R dist3(R dx, R dy, R dz) {
-    return std::sqrt(dx * dx + dy * dy + dz * dz);
+    return std::sqrt(std::fma(dz, dz, std::fma(dy, dy, dx * dx)));
 }
 
 // Coupling gate and particle -> wave volume flux (no reference counterpart):
