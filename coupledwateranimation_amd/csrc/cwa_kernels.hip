@@ -47,6 +47,7 @@ __global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+constexpr int kScanDirectTiles = 1024;  // tile prefixes summed in-kernel up to here
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
@@ -84,6 +85,32 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total) {
     return before + incl - v;
 }
 
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* __restrict__ in, int n,
+                                                             const int* __restrict__ tile_prefix,
+                                                             int* __restrict__ out,
+                                                             int* __restrict__ zero_out) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_excl_scan(s, total) + tile_prefix[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        if (i < n) {
+            out[i] = run;
+            if (zero_out) zero_out[i] = 0;
+        }
+        run += v[k];
+    }
+}
+
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int* __restrict__ in, int n,
                                                               int* __restrict__ part) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -104,13 +131,24 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int* __restr
     }
 }
 
-// Exclusive scan of one tile per block, each tile offset by tile_prefix
-// (null: 0).  When total_out is given, block 0 of a single-tile launch
-// writes the grand total there.
+// Exclusive scan of one tile per block.  The tile's prefix is the sum of
+// the tile sums before it (part[0 .. blockIdx.x), summed by the block
+// itself; null part: single tile).  total_out (optional) receives the grand
+// total, written by the last block.  zero_out (optional) is cleared over
+// the same index range (the other half of a double-buffered counter).
 __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int* __restrict__ in, int n,
-                                                            const int* __restrict__ tile_prefix,
+                                                            const int* __restrict__ part,
                                                             int* __restrict__ out,
-                                                            int* __restrict__ total_out) {
+                                                            int* __restrict__ total_out,
+                                                            int* __restrict__ zero_out) {
+    int prefix = 0;
+    if (part) {
+        int s = 0;
+        for (int t = threadIdx.x; t < (int)blockIdx.x; t += kScanThreads) s += part[t];
+        int tot;
+        (void)block_excl_scan(s, tot);
+        prefix = tot;
+    }
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
@@ -121,14 +159,17 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int* __restric
         s += v[k];
     }
     int total;
-    int run = block_excl_scan(s, total) + (tile_prefix ? tile_prefix[blockIdx.x] : 0);
+    int run = block_excl_scan(s, total) + prefix;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int i = base + k;
-        if (i < n) out[i] = run;
+        if (i < n) {
+            out[i] = run;
+            if (zero_out) zero_out[i] = 0;
+        }
         run += v[k];
     }
-    if (total_out && threadIdx.x == 0) *total_out = total + (tile_prefix ? tile_prefix[blockIdx.x] : 0);
+    if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = total + prefix;
 }
 
 __global__ void k_scan_total(const int* __restrict__ src, int* __restrict__ dst) {
@@ -140,31 +181,38 @@ __global__ void k_scan_total(const int* __restrict__ src, int* __restrict__ dst)
 size_t scan_workspace_ints(int n) {
     size_t ws = 0;
     int m = n;
-    while (m > kScanTile) {
+    while (m > kScanTile * kScanDirectTiles) {
         const int tiles = (m + kScanTile - 1) / kScanTile;
         ws += 2 * (size_t)tiles + 1;
         m = tiles;
     }
-    return ws + 1;
+    return ws + (size_t)(m + kScanTile - 1) / kScanTile + 1;
 }
 
 // out has n+1 entries; out[n] = total.  ws holds scan_workspace_ints(n).
-hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st) {
+// Up to kScanDirectTiles tiles the tile prefixes are summed inside
+// k_scan_down (two launches); larger inputs recurse on the tile sums.
+hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st,
+                       int* zero_out) {
     if (n <= 0) {
         k_fill_i32<<<1, 64, 0, st>>>(out, 1, 0);
         return hipGetLastError();
     }
-    if (n <= kScanTile) {
-        k_scan_down<<<1, kScanThreads, 0, st>>>(in, n, nullptr, out, out + n);
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles == 1) {
+        k_scan_down<<<1, kScanThreads, 0, st>>>(in, n, nullptr, out, out + n, zero_out);
         return hipGetLastError();
     }
-    const int tiles = (n + kScanTile - 1) / kScanTile;
     int* part = ws;
-    int* part_ex = ws + tiles;  // tiles + 1 entries
     k_scan_reduce<<<tiles, kScanThreads, 0, st>>>(in, n, part);
-    hipError_t e = launch_scan(part, part_ex, tiles, ws + 2 * (size_t)tiles + 1, st);
+    if (tiles <= kScanDirectTiles) {
+        k_scan_down<<<tiles, kScanThreads, 0, st>>>(in, n, part, out, out + n, zero_out);
+        return hipGetLastError();
+    }
+    int* part_ex = ws + tiles;  // tiles + 1 entries
+    hipError_t e = launch_scan(part, part_ex, tiles, ws + 2 * (size_t)tiles + 1, st, nullptr);
     if (e != hipSuccess) return e;
-    k_scan_down<<<tiles, kScanThreads, 0, st>>>(in, n, part_ex, out, nullptr);
+    k_scan_apply<<<tiles, kScanThreads, 0, st>>>(in, n, part_ex, out, zero_out);
     k_scan_total<<<1, 64, 0, st>>>(part_ex + tiles, out + n);
     return hipGetLastError();
 }
@@ -191,21 +239,21 @@ __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ p
         key[i] = c;
     }
     // Wave-aggregated atomics: the slots of a wave are nearly sorted by cell,
-    // so lanes sharing a cell take one atomicAdd per group (leader = lowest
-    // lane) and number themselves by lane order inside the group.
-    unsigned long long pending = __ballot(valid);
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    int rank = 0;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int lk = __shfl(c, leader, 64);
-        const unsigned long long grp = __ballot(valid && c == lk);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[lk], __popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (valid && c == lk) rank = base + __popcll(grp & lt);
-        pending &= ~grp;
-    }
+    // so each run of equal keys (in lane order) takes one atomicAdd for its
+    // whole length, issued by the run's first lane; every run of the wave
+    // issues in the same instruction.  A lane's provisional rank is the run
+    // base plus its distance from the run head.
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || c != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long after = heads & ~upto;
+    const int run_end = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && valid) base = atomicAdd(&count[c], run_end - lane);
+    const int my_head = 63 - __clzll((long long)(heads & upto));
+    base = __shfl(base, my_head, 64);
+    const int rank = base + (lane - my_head);
     if (valid) trank[i] = rank;
 }
 
@@ -291,10 +339,12 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
                                                  float* __restrict__ rho_w,
                                                  float2* __restrict__ pw,
                                                  float* __restrict__ flux,
-                                                 uint2* __restrict__ nbmask) {
+                                                 uint2* __restrict__ nbmask,
+                                                 float4* __restrict__ nrec) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
+    const float4 vi = vel[i];
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
     const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
@@ -334,48 +384,58 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
         rho += wf / P.k_rp;
     }
     rho = fmaxf(P.rho0, rho);
+    const float irho = 1.0f / rho;
     rho_w[i] = rho;
-    pw[i] = make_float2(pressure, 1.0f / rho);
+    pw[i] = make_float2(pressure, irho);
+    // the 32-byte record the force pass reads for this particle as a neighbour
+    nrec[2 * (size_t)i] = make_float4(pi.x, pi.y, pi.z, pressure);
+    nrec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, irho);
     float fl = 0.0f;
     if (P.band > 0.0f) {
         const float d = fabsf(pi.y - height);
         if (d < P.band) {
             const float qq = d / P.band;
             const float w = (1.0f - qq * qq) * (1.0f - qq * qq);
-            fl = P.m_rho0 * vel[i].y * w;
+            fl = P.m_rho0 * vi.y * w;
         }
     }
     flux[i] = fl;
 }
 
-// Pressure + viscosity contribution of neighbour j (force_comp.glsl:85-86).
+// Pressure + viscosity contribution of one neighbour (force_comp.glsl:85-86).
 struct PairAcc {
     float pfx, pfy, pfz, vfx, vfy, vfz;
 };
 
+// Neighbour record written by the density pass: (x, y, z, p), (vx, vy, vz, 1/rho),
+// one 32-byte block per particle so a neighbour costs one cache line.
+struct Nbr {
+    float4 a;
+    float4 b;
+};
+
+__device__ __forceinline__ Nbr load_nbr(const float4* __restrict__ nrec, int j) {
+    return Nbr{nrec[2 * (size_t)j], nrec[2 * (size_t)j + 1]};
+}
+
 __device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4 vi, float p_i,
-                                           float m_spiky_half, float m_lap,
-                                           const float4* __restrict__ pos,
-                                           const float4* __restrict__ vel,
-                                           const float2* __restrict__ pw, int j, PairAcc& a) {
-    const float4 pj = pos[j];
-    const float2 q = pw[j];
-    const float4 vj = vel[j];
-    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                                           float m_spiky_half, float m_lap, const Nbr& nb,
+                                           PairAcc& a) {
+    const float dx = pi.x - nb.a.x, dy = pi.y - nb.a.y, dz = pi.z - nb.a.z;
     const float r2 = dist2_fma(dx, dy, dz);
     // an exactly coincident pair (possible after boundary clamping) has no
     // pressure direction: rinv = 0 drops that term
     const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float r = r2 * rinv;
     const float hr = P.h - r;
-    const float s = m_spiky_half * (p_i + q.x) * q.y * hr * hr * rinv;
+    const float s = m_spiky_half * (p_i + nb.a.w) * nb.b.w * hr * hr * rinv;
     a.pfx -= s * dx;
     a.pfy -= s * dy;
     a.pfz -= s * dz;
-    const float t = m_lap * q.y * hr;
-    a.vfx += t * (vj.x - vi.x);
-    a.vfy += t * (vj.y - vi.y);
-    a.vfz += t * (vj.z - vi.z);
+    const float t = m_lap * nb.b.w * hr;
+    a.vfx += t * (nb.b.x - vi.x);
+    a.vfy += t * (nb.b.y - vi.y);
+    a.vfz += t * (nb.b.z - vi.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -388,7 +448,8 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     DevParams P, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
     int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
-    const uint2* __restrict__ nbmask, const float* __restrict__ H0, float4* __restrict__ pos_o,
+    const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
+    const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
     float* __restrict__ rho_o, float* __restrict__ prs_o) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -407,22 +468,25 @@ __global__ void __launch_bounds__(256) k_force_integrate(
         int b, e;
         column_range(P, offset, ix, iy, iz, c, b, e);
         const uint2 mk = nbmask[(size_t)c * n + i];
-        uint32_t lo = mk.x, hi = mk.y;
-        while (lo) {
-            const int j = b + __builtin_ctz(lo);
-            lo &= lo - 1u;
-            if (j != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
-        }
-        while (hi) {
-            const int j = b + 32 + __builtin_ctz(hi);
-            hi &= hi - 1u;
-            if (j != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
+        unsigned long long m = ((unsigned long long)mk.y << 32) | mk.x;
+        // two neighbours per iteration: both loads are issued before either
+        // pair is evaluated; pairs are still accumulated in bit order
+        while (m) {
+            const int j0 = b + __builtin_ctzll(m);
+            m &= m - 1ull;
+            const bool two = m != 0ull;
+            const int j1 = two ? b + __builtin_ctzll(m) : j0;
+            m &= two ? m - 1ull : m;
+            const Nbr n0 = load_nbr(nrec, j0);
+            const Nbr n1 = load_nbr(nrec, j1);
+            if (j0 != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, n0, a);
+            if (two && j1 != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, n1, a);
         }
         for (int j = b + 64; j < e; ++j) {  // columns longer than 64 candidates
             const float4 pj = pos[j];
             const float r2 = dist2_fma(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             if (r2 < P.r2_lt && j != i)
-                force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, pos, vel, pw, j, a);
+                force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, load_nbr(nrec, j), a);
         }
     }
     float vfx = a.vfx, vfy = a.vfy, vfz = a.vfz;
@@ -670,20 +734,20 @@ hipError_t launch_stable_reorder(const int* key, const int* offset, const int* t
 
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
-                          hipStream_t st) {
+                          float4* nrec, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_density<<<blocks_for(n, 256), 256, 0, st>>>(P, B.pos, B.vel, offset, n, H0, rho_w, pw,
-                                                  flux, nbmask);
+                                                  flux, nbmask, nrec);
     return hipGetLastError();
 }
 
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const uint2* nbmask, const float* H0, ParticleArrays& A,
-                                  float* rho, float* prs, hipStream_t st) {
+                                  const uint2* nbmask, const float4* nrec, const float* H0,
+                                  ParticleArrays& A, float* rho, float* prs, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P, dt, B.pos, B.vel, B.force, B.id,
-                                                          offset, n, rho_w, pw, nbmask, H0,
+                                                          offset, n, rho_w, pw, nbmask, nrec, H0,
                                                           A.pos, A.vel, A.force, A.id, rho, prs);
     return hipGetLastError();
 }

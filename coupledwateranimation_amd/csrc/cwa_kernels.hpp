@@ -23,7 +23,9 @@ struct ParticleArrays {
 };
 
 size_t scan_workspace_ints(int n);
-hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st);
+// zero_out (optional, n ints) is cleared in the same pass.
+hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st,
+                       int* zero_out = nullptr);
 hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st);
 hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* trank,
                              int* count, hipStream_t st);
@@ -33,14 +35,15 @@ hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offse
 hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
                                  int* list, const ParticleArrays* in, ParticleArrays* out,
                                  hipStream_t st);
-// nbmask: 9*n uint2, written by the density pass, read by the force pass
+// nbmask (9*n uint2) and nrec (2*n float4) are written by the density pass
+// and read by the force pass
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
-                          hipStream_t st);
+                          float4* nrec, hipStream_t st);
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const uint2* nbmask, const float* H0, ParticleArrays& A,
-                                  float* rho, float* prs, hipStream_t st);
+                                  const uint2* nbmask, const float4* nrec, const float* H0,
+                                  ParticleArrays& A, float* rho, float* prs, hipStream_t st);
 hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
                               float* colflux, hipStream_t st);
 hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,

@@ -118,13 +118,19 @@ class UniformGrid3D {
     const GridDesc& desc() const { return g_; }
     int num_cells() const { return g_.dims[0] * g_.dims[1] * g_.dims[2]; }
     int* key() const { return key_.get(); }
-    int* count() const { return count_.get(); }
+    // counts of the last build (the other half of the double buffer is the
+    // zeroed counter of the next build)
+    int* count() const { return count_[last_].get(); }
     int* offset() const { return offset_.get(); }
     int* list() const { return list_.get(); }
+    // Zero both counters (the counter of the next build must start at zero;
+    // each build's scan clears the other one).
+    void ClearCounters(hipStream_t st);
 
    private:
     GridDesc g_{};
-    DeviceBuffer<int> key_, trank_, count_, tmp_, list_, offset_, ws_;
+    DeviceBuffer<int> key_, trank_, count_[2], tmp_, list_, offset_, ws_;
+    int cur_ = 0, last_ = 0;
 };
 
 // StencilImage2DTripleBuffered: three fp32 fields rotated by PingPong.
@@ -171,10 +177,12 @@ class SphParticles3D {
     float2* pw() const { return pw_.get(); }
     float* flux() const { return flux_.get(); }
     uint2* nbmask() const { return nbmask_.get(); }
+    float4* nrec() const { return nrec_.get(); }
 
    private:
     int n_ = 0;
     DeviceBuffer<uint2> nbmask_;  // 9 x n neighbour bitmasks of the last step
+    DeviceBuffer<float4> nrec_;   // 2 x n neighbour records of the last step
     DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
     DeviceBuffer<int> id_[2];
     DeviceBuffer<float> rho_, prs_, rho_w_, flux_;

@@ -65,9 +65,16 @@ void UniformGrid3D::Init(const GridDesc& g, int max_particles) {
     trank_.resize(np);
     tmp_.resize(np);
     list_.resize(np);
-    count_.resize(nc);
+    count_[0].resize(nc);
+    count_[1].resize(nc);
     offset_.resize(nc + 1);
     ws_.resize(scan_workspace_ints((int)nc));
+    cur_ = last_ = 0;
+}
+
+void UniformGrid3D::ClearCounters(hipStream_t st) {
+    for (auto& c : count_) hip_check(launch_fill_i32(c.get(), num_cells(), 0, st), "clear counts");
+    hip_check(launch_fill_i32(offset_.get(), num_cells() + 1, 0, st), "clear offsets");
 }
 
 void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in,
@@ -76,15 +83,15 @@ void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in
     auto mark = [&](int slot, int which) {
         if (ev) hip_check(hipEventRecord(ev[2 * slot + which], st), "event");
     };
-    mark(K_CLEAR, 0);
-    hip_check(launch_fill_i32(count_.get(), nc, 0, st), "clear counts");
-    mark(K_CLEAR, 1);
+    int* cnt = count_[cur_].get();  // zero: cleared by the previous build's scan
     mark(K_HASH, 0);
-    hip_check(launch_hash_count(pos, n, g_, key_.get(), trank_.get(), count_.get(), st), "hash");
+    hip_check(launch_hash_count(pos, n, g_, key_.get(), trank_.get(), cnt, st), "hash");
     mark(K_HASH, 1);
     mark(K_SCAN, 0);
-    hip_check(launch_scan(count_.get(), offset_.get(), nc, ws_.get(), st), "scan");
+    hip_check(launch_scan(cnt, offset_.get(), nc, ws_.get(), st, count_[cur_ ^ 1].get()), "scan");
     mark(K_SCAN, 1);
+    last_ = cur_;
+    cur_ ^= 1;
     mark(K_SCATTER, 0);
     hip_check(launch_scatter_tmp(key_.get(), trank_.get(), offset_.get(), n, tmp_.get(), st),
               "scatter");
@@ -179,6 +186,7 @@ void SphParticles3D::Allocate(int n) {
     flux_.resize(m);
     pw_.resize(m);
     nbmask_.resize(9 * m);
+    nrec_.resize(2 * m);
     A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
     B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
 }
@@ -363,8 +371,7 @@ void CoupledScene::Reinit(const cwa_init& init) {
     particles.Allocate(init.n);
     particles.Upload(init.pos, init.vel, init.force, st_);
     grid.Init(grid_desc(params_), init.n);
-    hip_check(launch_fill_i32(grid.count(), grid.num_cells(), 0, st_), "count zero");
-    hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
+    grid.ClearCounters(st_);
     colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
     hip_check(launch_fill_i32(reinterpret_cast<int*>(colflux.get()), (int)colflux.size(), 0, st_),
               "colflux zero");
@@ -388,8 +395,7 @@ void CoupledScene::SetParams(const cwa_params& p) {
     params_ = p;
     if (grid_changed) {
         grid.Init(grid_desc(params_), particles.n());
-        hip_check(launch_fill_i32(grid.count(), grid.num_cells(), 0, st_), "count zero");
-        hip_check(launch_fill_i32(grid.offset(), grid.num_cells() + 1, 0, st_), "offset zero");
+        grid.ClearCounters(st_);
         colflux.resize((size_t)params_.grid_dims[0] * params_.grid_dims[2]);
     }
     hip_check(launch_fill_i32(reinterpret_cast<int*>(colflux.get()), (int)colflux.size(), 0, st_),
@@ -409,13 +415,14 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2]) {
                        ev ? &ev[0][0] : nullptr);
         mark(K_DENSITY, 0);
         hip_check(launch_density(dp_, particles.B, grid.offset(), n, H0, particles.rho_w(),
-                                 particles.pw(), particles.flux(), particles.nbmask(), st_),
+                                 particles.pw(), particles.flux(), particles.nbmask(),
+                                 particles.nrec(), st_),
                   "density");
         mark(K_DENSITY, 1);
         mark(K_FORCE, 0);
         hip_check(launch_force_integrate(dp_, dt, particles.B, grid.offset(), n, particles.rho_w(),
-                                         particles.pw(), particles.nbmask(), H0, particles.A,
-                                         particles.rho(), particles.prs(), st_),
+                                         particles.pw(), particles.nbmask(), particles.nrec(), H0,
+                                         particles.A, particles.rho(), particles.prs(), st_),
                   "force/integrate");
         mark(K_FORCE, 1);
         if (couple) {
@@ -458,7 +465,7 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
         for (int k = 0; k < nsteps; ++k) Step(dt);
         return;
     }
-    const int cycle = 3;  // wave triple buffer period (particles: period 1)
+    const int cycle = 6;  // wave triple buffer (3) x grid counter double buffer (2)
     if (nsteps >= cycle && (!gexec_ || graph_dt_ != dt)) {
         invalidate_graph();
         hipGraph_t g = nullptr;
@@ -782,7 +789,7 @@ int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times
             hip_check(hipStreamSynchronize(S.stream()), "sync");
             for (int slot = 0; slot < K_NUM; ++slot) {
                 const bool ran = slot == K_WAVE ||
-                                 (S.particles.n() > 0 && slot <= K_FORCE) ||
+                                 (S.particles.n() > 0 && slot >= K_HASH && slot <= K_FORCE) ||
                                  (slot == K_COLFLUX && couple);
                 if (!ran) continue;
                 float ms = 0.0f;

@@ -219,9 +219,10 @@ int cwa_scene_step(cwa_scene* s, float dt, int nsteps);
 /* Same as cwa_scene_step with HIP events around every kernel; `out`
  * receives the per-kernel average device times. */
 int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times* out);
-/* Replay-based execution: capture a 6-step cycle (the period of the
- * particle double buffer x the wave triple buffer) into a hipGraph and
- * launch it; nsteps that are not a multiple of 6 run the tail eagerly. */
+/* Replay-based execution: capture a 6-step cycle (the period of the grid
+ * counter double buffer x the wave triple buffer) into a hipGraph and
+ * launch it; nsteps that are not a multiple of 6 run the tail eagerly.
+ * The graph is re-captured when dt changes. */
 int cwa_scene_step_graph(cwa_scene* s, float dt, int nsteps);
 int cwa_scene_sync(cwa_scene* s);
 /* The scene's hipStream_t (as void*). */
