@@ -96,7 +96,12 @@ class KernelTimes(C.Structure):
     ]
 
 
-STRUCTS = [Params, Init, ParticleView, GridView, HostState, KernelTimes]
+class Grad(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("visc", "gas_const", "rest_rho", "lambda_x",
+                                          "lambda_z", "atten", "couple_strength", "beta")]
+
+
+STRUCTS = [Params, Init, ParticleView, GridView, HostState, KernelTimes, Grad]
 
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
@@ -130,6 +135,12 @@ SIGNATURES = {
     "cwa_grid_build": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_float,
                                  C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
+    "cwa_tape_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "cwa_tape_forward": (C.c_int, [C.c_void_p, C.c_float]),
+    "cwa_tape_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(Grad),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cwa_tape_destroy": (C.c_int, [C.c_void_p]),
     "cwa_gen_lattice": (C.c_int, [C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float,
                                   C.POINTER(C.c_int32), C.c_uint64]),

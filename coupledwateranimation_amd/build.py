@@ -19,8 +19,8 @@ OUT = PKG / "_build"
 LIB = OUT / "libcwa.so"
 ROOT = PKG.parent
 
-SOURCES = ["cwa_kernels.hip", "cwa_scene.hip", "cwa_gen.cpp"]
-HEADERS = ["cwa_device.hpp", "cwa_kernels.hpp", "cwa_modules.hpp"]
+SOURCES = ["cwa_kernels.hip", "cwa_adjoint.hip", "cwa_scene.hip", "cwa_tape.hip", "cwa_gen.cpp"]
+HEADERS = ["cwa_device.hpp", "cwa_kernels.hpp", "cwa_adjoint.hpp", "cwa_modules.hpp"]
 
 ARCH = os.environ.get("CWA_OFFLOAD_ARCH", "gfx950")
 

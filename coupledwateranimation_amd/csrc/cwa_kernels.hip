@@ -340,7 +340,8 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
                                                  float2* __restrict__ pw,
                                                  float* __restrict__ flux,
                                                  uint2* __restrict__ nbmask,
-                                                 float4* __restrict__ nrec) {
+                                                 float4* __restrict__ nrec,
+                                                 float* __restrict__ rho_raw) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
@@ -374,6 +375,7 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
         nbmask[(size_t)c * n + i] = make_uint2(lo, hi);
     }
     float rho = P.m_poly6 * acc;
+    if (rho_raw) rho_raw[i] = rho;  // taped for the adjoint
     float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
     const float u = (pi.x - P.org[0]) * P.uvs[0];
     const float v = (pi.z - P.org[1]) * P.uvs[1];
@@ -451,7 +453,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
     const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
-    float* __restrict__ rho_o, float* __restrict__ prs_o) {
+    float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
@@ -531,6 +533,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     float Fy = a.pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
     float Fz = a.pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
     float Fw = fp.w;
+    if (fpre) fpre[i] = make_float4(Fx, Fy, Fz, 0.0f);  // taped for the adjoint
 
     // integrate_comp.glsl:56-92
     float nvx = vi.x + dt * (Fx / rho), nvy = vi.y + dt * (Fy / rho), nvz = vi.z + dt * (Fz / rho);
@@ -734,21 +737,23 @@ hipError_t launch_stable_reorder(const int* key, const int* offset, const int* t
 
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
-                          float4* nrec, hipStream_t st) {
+                          float4* nrec, float* rho_raw, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_density<<<blocks_for(n, 256), 256, 0, st>>>(P, B.pos, B.vel, offset, n, H0, rho_w, pw,
-                                                  flux, nbmask, nrec);
+                                                  flux, nbmask, nrec, rho_raw);
     return hipGetLastError();
 }
 
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
                                   const uint2* nbmask, const float4* nrec, const float* H0,
-                                  ParticleArrays& A, float* rho, float* prs, hipStream_t st) {
+                                  ParticleArrays& A, float* rho, float* prs, float4* fpre,
+                                  hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P, dt, B.pos, B.vel, B.force, B.id,
                                                           offset, n, rho_w, pw, nbmask, nrec, H0,
-                                                          A.pos, A.vel, A.force, A.id, rho, prs);
+                                                          A.pos, A.vel, A.force, A.id, rho, prs,
+                                                          fpre);
     return hipGetLastError();
 }
 

@@ -36,14 +36,17 @@ hipError_t launch_stable_reorder(const int* key, const int* offset, const int* t
                                  int* list, const ParticleArrays* in, ParticleArrays* out,
                                  hipStream_t st);
 // nbmask (9*n uint2) and nrec (2*n float4) are written by the density pass
-// and read by the force pass
+// and read by the force pass.  rho_raw (density before the wave term and the
+// rest-density clamp) and fpre (force before the foam branch) are optional
+// outputs for the adjoint tape (null: not written).
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
-                          float4* nrec, hipStream_t st);
+                          float4* nrec, float* rho_raw, hipStream_t st);
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
                                   const uint2* nbmask, const float4* nrec, const float* H0,
-                                  ParticleArrays& A, float* rho, float* prs, hipStream_t st);
+                                  ParticleArrays& A, float* rho, float* prs, float4* fpre,
+                                  hipStream_t st);
 hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
                               float* colflux, hipStream_t st);
 hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,

@@ -68,6 +68,14 @@ class DeviceBuffer {
     }
     T* get() const { return p_; }
     size_t size() const { return n_; }
+    void swap(DeviceBuffer& o) noexcept {
+        T* p = p_;
+        p_ = o.p_;
+        o.p_ = p;
+        const size_t n = n_;
+        n_ = o.n_;
+        o.n_ = n;
+    }
 
    private:
     T* p_ = nullptr;
@@ -126,6 +134,7 @@ class UniformGrid3D {
     // Zero both counters (the counter of the next build must start at zero;
     // each build's scan clears the other one).
     void ClearCounters(hipStream_t st);
+    int parity() const { return cur_; }
 
    private:
     GridDesc g_{};
@@ -151,6 +160,8 @@ class HeightField2DTripleBuffered {
     void Evolve(const DevParams& P, const float* colflux, float dt, hipStream_t st);
     // PingPong (.cpp:33-40): written image becomes read 0.
     void PingPong();
+    // rotation state (which buffer is read 0 / read 1 / write), 0..2
+    int phase() const { return read_[0]; }
     float* GetReadImage(int i) const { return img_[read_[i]].get(); }
     float* GetWriteImage() const { return img_[write_].get(); }
     int width() const { return w_; }
@@ -189,6 +200,12 @@ class SphParticles3D {
     DeviceBuffer<float2> pw_;
 };
 
+// Optional per-step outputs recorded for the adjoint tape.
+struct TapeOut {
+    float* rho_raw = nullptr;  // density before the wave term and the clamp
+    float4* fpre = nullptr;    // force before the foam branch
+};
+
 // Kernel slots of cwa_kernel_times.
 enum KernelSlot {
     K_CLEAR = 0,
@@ -217,8 +234,9 @@ class CoupledScene : public Module {
     void Reinit(const cwa_init& init);
     void SetParams(const cwa_params& p);
     const cwa_params& params() const { return params_; }
-    // idle() (Main.cpp:530-562).  events: optional [K_NUM][2] per step.
-    void Step(float dt, hipEvent_t (*events)[2] = nullptr);
+    // idle() (Main.cpp:530-562).  events: optional [K_NUM][2] per step;
+    // tape: optional extra outputs for the adjoint.
+    void Step(float dt, hipEvent_t (*events)[2] = nullptr, const TapeOut* tape = nullptr);
     void StepGraph(float dt, int nsteps);
     void WaveOnly(float dt);
     void WaveReinit();
@@ -226,6 +244,10 @@ class CoupledScene : public Module {
     void CheckFinite();
     void Readback(cwa_host_state* h);
     hipStream_t stream() const { return st_; }
+    const DevParams& devparams() const { return dp_; }
+    // Host-side phase of the buffer rotations (wave triple buffer x grid
+    // counter double buffer), period 6.
+    int phase() const { return wave.phase() * 2 + grid.parity(); }
 
     SphParticles3D particles;
     UniformGrid3D grid;
@@ -240,6 +262,7 @@ class CoupledScene : public Module {
     hipStream_t st_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = 0.0f;
+    int graph_phase_ = -1;
     float dt_ = 0.0f;
     DeviceBuffer<int> flag_;
 };

@@ -403,7 +403,7 @@ void CoupledScene::SetParams(const cwa_params& p) {
     build_devparams();
 }
 
-void CoupledScene::Step(float dt, hipEvent_t (*ev)[2]) {
+void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
     auto mark = [&](int slot, int which) {
         if (ev) hip_check(hipEventRecord(ev[slot][which], st_), "event");
     };
@@ -416,13 +416,14 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2]) {
         mark(K_DENSITY, 0);
         hip_check(launch_density(dp_, particles.B, grid.offset(), n, H0, particles.rho_w(),
                                  particles.pw(), particles.flux(), particles.nbmask(),
-                                 particles.nrec(), st_),
+                                 particles.nrec(), tape ? tape->rho_raw : nullptr, st_),
                   "density");
         mark(K_DENSITY, 1);
         mark(K_FORCE, 0);
         hip_check(launch_force_integrate(dp_, dt, particles.B, grid.offset(), n, particles.rho_w(),
                                          particles.pw(), particles.nbmask(), particles.nrec(), H0,
-                                         particles.A, particles.rho(), particles.prs(), st_),
+                                         particles.A, particles.rho(), particles.prs(),
+                                         tape ? tape->fpre : nullptr, st_),
                   "force/integrate");
         mark(K_FORCE, 1);
         if (couple) {
@@ -466,7 +467,16 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
         return;
     }
     const int cycle = 6;  // wave triple buffer (3) x grid counter double buffer (2)
-    if (nsteps >= cycle && (!gexec_ || graph_dt_ != dt)) {
+    int done = 0;
+    // A captured graph replays the buffers of the phase it was captured at:
+    // step eagerly until the rotation phase matches it again.
+    if (gexec_ && graph_dt_ == dt) {
+        while (done < nsteps && phase() != graph_phase_) {
+            Step(dt);
+            ++done;
+        }
+    }
+    if (nsteps - done >= cycle && (!gexec_ || graph_dt_ != dt)) {
         invalidate_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
@@ -482,9 +492,9 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
         (void)hipGraphDestroy(g);
         hip_check(e, "graph instantiate");
         graph_dt_ = dt;
+        graph_phase_ = phase();  // capture advanced the host state a full cycle
     }
-    int done = 0;
-    while (gexec_ && nsteps - done >= cycle) {
+    while (gexec_ && graph_dt_ == dt && phase() == graph_phase_ && nsteps - done >= cycle) {
         hip_check(hipGraphLaunch(gexec_, st_), "graph launch");
         done += cycle;
     }
@@ -571,6 +581,16 @@ static int guarded(F&& f) {
     }
 }
 
+namespace cwa {
+CoupledScene* scene_impl(cwa_scene* s) {
+    if (!s) throw Error(CWA_EINVAL, "null scene");
+    return s->impl;
+}
+int guarded_call(void (*fn)(void*), void* ctx) {
+    return guarded([&] { fn(ctx); });
+}
+}  // namespace cwa
+
 #define CWA_REQUIRE(cond, msg)                  \
     do {                                        \
         if (!(cond)) throw Error(CWA_EINVAL, msg); \
@@ -588,6 +608,7 @@ int cwa_struct_size(int which) {
         case 3: return (int)sizeof(cwa_grid_view);
         case 4: return (int)sizeof(cwa_host_state);
         case 5: return (int)sizeof(cwa_kernel_times);
+        case 6: return (int)sizeof(cwa_grad);
         default: return -1;
     }
 }

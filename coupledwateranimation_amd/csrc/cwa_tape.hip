@@ -1,0 +1,341 @@
+// cwa_tape.hip — checkpointed forward + adjoint sweep over many coupled
+// steps (the gradient path), and its C ABI.
+//
+// Forward: the scene's state is saved every `ckpt_every` steps (particle
+// SoA + id + the two newest heightfields) and once more at the end.
+// Backward: for each segment, last to first, the segment's checkpoint is
+// restored and its steps re-run with every intermediate the adjoint reads
+// recorded per step (sorted state, permutation, offsets, neighbour masks,
+// densities, pressures, neighbour records, pre-foam force, heightfields,
+// column flux); the adjoint then walks the segment's steps in reverse.
+// Memory: checkpoints of the whole run plus the tape of one segment.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/cwa.h"
+#include "cwa_adjoint.hpp"
+#include "cwa_modules.hpp"
+
+namespace cwa {
+
+class Tape {
+   public:
+    Tape(CoupledScene& s, int nsteps, int every);
+    void Forward(float dt);
+    void Backward(const float* dH0, const float* dH1, const float* dpos, const float* dvel,
+                  double* grad, float* dpos0, float* dvel0);
+
+   private:
+    struct Ckpt {
+        DeviceBuffer<float4> pos, vel, force;
+        DeviceBuffer<int> id;
+        DeviceBuffer<float> h0, h1;
+    };
+    struct StepTape {
+        DeviceBuffer<float4> pos, vel, nrec, fpre;
+        DeviceBuffer<int> list, offset;
+        DeviceBuffer<uint2> nbmask;
+        DeviceBuffer<float> rho, rho_raw, h0, h1, colflux;
+        DeviceBuffer<float2> pw;
+    };
+    void save(Ckpt& c);
+    void restore(const Ckpt& c);
+    void record_step(StepTape& t);
+    void adjoint_step(const StepTape& t);
+    template <class T>
+    void copy(T* dst, const T* src, size_t count) {
+        if (count)
+            hip_check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, st_),
+                      "tape copy");
+    }
+
+    CoupledScene& S_;
+    hipStream_t st_;
+    int nsteps_, every_, nseg_;
+    int n_, W_, H_, ncell_, ncol_;
+    float dt_ = 0.0f;
+    bool forward_done_ = false;
+    std::vector<Ckpt> ck_;
+    std::vector<StepTape> tp_;
+    DeviceBuffer<float4> xb_, vb_, xb2_, vb2_, xl_, vl_, adjrec_;
+    DeviceBuffer<float> G0_, G1_, G0n_, G1n_, rhob_, rtb_, Sbar_;
+    DeviceBuffer<double> pgrad_;
+};
+
+Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
+    if (nsteps < 1) throw Error(CWA_EINVAL, "nsteps must be >= 1");
+    if (every < 1) throw Error(CWA_EINVAL, "ckpt_every must be >= 1");
+    if (s.params().torque_coeff != 0.0f)
+        throw Error(CWA_EINVAL, "the adjoint requires torque_coeff == 0");
+    nsteps_ = nsteps;
+    every_ = every;
+    nseg_ = (nsteps + every - 1) / every;
+    n_ = s.particles.n();
+    W_ = s.wave.width();
+    H_ = s.wave.height();
+    ncell_ = s.grid.num_cells();
+    ncol_ = s.params().grid_dims[0] * s.params().grid_dims[2];
+    const size_t n = (size_t)std::max(n_, 1), tex = (size_t)W_ * H_;
+    ck_.resize(nseg_ + 1);
+    for (auto& c : ck_) {
+        c.pos.resize(n);
+        c.vel.resize(n);
+        c.force.resize(n);
+        c.id.resize(n);
+        c.h0.resize(tex);
+        c.h1.resize(tex);
+    }
+    tp_.resize(std::min(every, nsteps));
+    for (auto& t : tp_) {
+        t.pos.resize(n);
+        t.vel.resize(n);
+        t.nrec.resize(2 * n);
+        t.fpre.resize(n);
+        t.list.resize(n);
+        t.offset.resize((size_t)ncell_ + 1);
+        t.nbmask.resize(9 * n);
+        t.rho.resize(n);
+        t.rho_raw.resize(n);
+        t.pw.resize(n);
+        t.h0.resize(tex);
+        t.h1.resize(tex);
+        t.colflux.resize(ncol_);
+    }
+    for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
+    adjrec_.resize(2 * n);
+    for (auto* b : {&G0_, &G1_, &G0n_, &G1n_}) b->resize(tex);
+    rhob_.resize(n);
+    rtb_.resize(n);
+    Sbar_.resize(ncol_);
+    pgrad_.resize(GRAD_NUM);
+}
+
+void Tape::save(Ckpt& c) {
+    const auto& A = S_.particles.A;
+    copy(c.pos.get(), A.pos, n_);
+    copy(c.vel.get(), A.vel, n_);
+    copy(c.force.get(), A.force, n_);
+    copy(c.id.get(), A.id, n_);
+    copy(c.h0.get(), S_.wave.GetReadImage(0), (size_t)W_ * H_);
+    copy(c.h1.get(), S_.wave.GetReadImage(1), (size_t)W_ * H_);
+}
+
+void Tape::restore(const Ckpt& c) {
+    auto& A = S_.particles.A;
+    copy(A.pos, c.pos.get(), n_);
+    copy(A.vel, c.vel.get(), n_);
+    copy(A.force, c.force.get(), n_);
+    copy(A.id, c.id.get(), n_);
+    copy(S_.wave.GetReadImage(0), c.h0.get(), (size_t)W_ * H_);
+    copy(S_.wave.GetReadImage(1), c.h1.get(), (size_t)W_ * H_);
+    S_.grid.ClearCounters(st_);
+}
+
+void Tape::Forward(float dt) {
+    dt_ = dt;
+    int done = 0;
+    for (int seg = 0; seg < nseg_; ++seg) {
+        save(ck_[seg]);
+        const int len = std::min(every_, nsteps_ - done);
+        for (int k = 0; k < len; ++k) S_.Step(dt);
+        done += len;
+    }
+    save(ck_[nseg_]);
+    hip_check(hipStreamSynchronize(st_), "tape forward");
+    forward_done_ = true;
+}
+
+// One step of the segment replay, recording what the adjoint reads.
+void Tape::record_step(StepTape& t) {
+    const size_t tex = (size_t)W_ * H_;
+    copy(t.h0.get(), S_.wave.GetReadImage(0), tex);
+    copy(t.h1.get(), S_.wave.GetReadImage(1), tex);
+    TapeOut out;
+    out.rho_raw = t.rho_raw.get();
+    out.fpre = t.fpre.get();
+    S_.Step(dt_, nullptr, &out);
+    auto& P = S_.particles;
+    copy(t.pos.get(), P.B.pos, n_);
+    copy(t.vel.get(), P.B.vel, n_);
+    copy(t.nrec.get(), P.nrec(), 2 * (size_t)n_);
+    copy(t.list.get(), S_.grid.list(), n_);
+    copy(t.offset.get(), S_.grid.offset(), (size_t)ncell_ + 1);
+    copy(t.nbmask.get(), P.nbmask(), 9 * (size_t)n_);
+    copy(t.rho.get(), P.rho_w(), n_);
+    copy(t.pw.get(), P.pw(), n_);
+    copy(t.colflux.get(), S_.colflux.get(), ncol_);
+}
+
+void Tape::adjoint_step(const StepTape& t) {
+    const DevParams& P = S_.devparams();
+    const bool couple = n_ > 0 && P.band > 0.0f;
+    hip_check(launch_adj_wave(P, dt_, t.h0.get(), t.h1.get(), couple ? t.colflux.get() : nullptr,
+                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), pgrad_.get(), st_),
+              "adj wave");
+    if (n_ > 0) {
+        if (couple)
+            hip_check(launch_adj_colflux(P, dt_, G0_.get(), Sbar_.get(), st_), "adj colflux");
+        hip_check(launch_adj_local(P, dt_, n_, t.pos.get(), t.vel.get(), t.fpre.get(),
+                                   t.rho.get(), t.h0.get(), xb_.get(), vb_.get(), G0n_.get(),
+                                   adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st_),
+                  "adj local");
+        hip_check(launch_adj_pairs(P, dt_, n_, t.pos.get(), t.vel.get(), t.offset.get(),
+                                   t.nbmask.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
+                                   t.rho_raw.get(), t.h0.get(), Sbar_.get(), adjrec_.get(),
+                                   rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
+                                   pgrad_.get(), st_),
+                  "adj pairs");
+        hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbmask.get(),
+                                       rtb_.get(), xl_.get(), vl_.get(), t.list.get(),
+                                       xb2_.get(), vb2_.get(), st_),
+                  "adj rho pairs");
+        xb_.swap(xb2_);
+        vb_.swap(vb2_);
+    }
+    G0_.swap(G0n_);
+    G1_.swap(G1n_);
+}
+
+void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const float* dvel,
+                    double* grad, float* dpos0, float* dvel0) {
+    if (!forward_done_) throw Error(CWA_ESTATE, "tape backward before forward");
+    const size_t tex = (size_t)W_ * H_;
+    // seeds: heights as given, particle seeds permuted into the final slots
+    std::vector<int> id(std::max(n_, 1));
+    if (n_ > 0)
+        hip_check(hipMemcpy(id.data(), ck_[nseg_].id.get(), n_ * sizeof(int),
+                            hipMemcpyDeviceToHost),
+                  "ids");
+    auto upload_particles = [&](const float* seed, DeviceBuffer<float4>& dst) {
+        std::vector<float> h(4 * (size_t)std::max(n_, 1), 0.0f);
+        if (seed)
+            for (int s = 0; s < n_; ++s)
+                for (int q = 0; q < 3; ++q) h[4 * (size_t)s + q] = seed[4 * (size_t)id[s] + q];
+        if (n_ > 0)
+            hip_check(hipMemcpy(dst.get(), h.data(), (size_t)n_ * sizeof(float4),
+                                hipMemcpyHostToDevice),
+                      "seed");
+    };
+    upload_particles(dpos, xb_);
+    upload_particles(dvel, vb_);
+    auto upload_field = [&](const float* seed, DeviceBuffer<float>& dst) {
+        if (seed)
+            hip_check(hipMemcpy(dst.get(), seed, tex * sizeof(float), hipMemcpyHostToDevice),
+                      "seed field");
+        else
+            hip_check(launch_fill_i32(reinterpret_cast<int*>(dst.get()), (int)tex, 0, st_),
+                      "zero field");
+    };
+    upload_field(dH0, G0_);
+    upload_field(dH1, G1_);
+    hip_check(hipMemsetAsync(pgrad_.get(), 0, GRAD_NUM * sizeof(double), st_), "grad zero");
+
+    for (int seg = nseg_ - 1; seg >= 0; --seg) {
+        restore(ck_[seg]);
+        const int len = std::min(every_, nsteps_ - seg * every_);
+        for (int k = 0; k < len; ++k) record_step(tp_[k]);
+        for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k]);
+    }
+    restore(ck_[nseg_]);  // leave the scene in its final state
+    hip_check(hipStreamSynchronize(st_), "backward");
+
+    double g[GRAD_NUM];
+    hip_check(hipMemcpy(g, pgrad_.get(), sizeof(g), hipMemcpyDeviceToHost), "grad");
+    for (int k = 0; k < GRAD_NUM; ++k) grad[k] = g[k];
+    // the adjoint state is now in the slot order of the initial state
+    std::vector<int> id0(std::max(n_, 1));
+    if (n_ > 0)
+        hip_check(hipMemcpy(id0.data(), ck_[0].id.get(), n_ * sizeof(int),
+                            hipMemcpyDeviceToHost),
+                  "ids0");
+    auto download = [&](const DeviceBuffer<float4>& src, float* out) {
+        if (!out || n_ == 0) return;
+        std::vector<float4> h(n_);
+        hip_check(hipMemcpy(h.data(), src.get(), n_ * sizeof(float4), hipMemcpyDeviceToHost),
+                  "grad particles");
+        for (int s = 0; s < n_; ++s) {
+            float* o = out + 4 * (size_t)id0[s];
+            o[0] = h[s].x;
+            o[1] = h[s].y;
+            o[2] = h[s].z;
+            o[3] = 0.0f;
+        }
+    };
+    download(xb_, dpos0);
+    download(vb_, dvel0);
+}
+
+}  // namespace cwa
+
+using namespace cwa;
+
+struct cwa_tape {
+    Tape* impl;
+};
+
+// defined in cwa_scene.hip
+namespace cwa {
+CoupledScene* scene_impl(cwa_scene* s);
+int guarded_call(void (*fn)(void*), void* ctx);
+}  // namespace cwa
+
+namespace {
+template <class F>
+int guarded_lambda(F&& f) {
+    return cwa::guarded_call([](void* c) { (*static_cast<F*>(c))(); }, &f);
+}
+}  // namespace
+
+extern "C" {
+
+int cwa_tape_create(cwa_scene* s, int nsteps, int ckpt_every, cwa_tape** out) {
+    return guarded_lambda([&] {
+        if (!s || !out) throw Error(CWA_EINVAL, "null argument");
+        *out = nullptr;
+        auto* t = new cwa_tape{nullptr};
+        try {
+            t->impl = new Tape(*scene_impl(s), nsteps, ckpt_every);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+int cwa_tape_forward(cwa_tape* t, float dt) {
+    return guarded_lambda([&] {
+        if (!t) throw Error(CWA_EINVAL, "null argument");
+        t->impl->Forward(dt);
+    });
+}
+
+int cwa_tape_backward(cwa_tape* t, const float* dheight0, const float* dheight1,
+                      const float* dpos, const float* dvel, cwa_grad* grad, float* dpos0,
+                      float* dvel0) {
+    return guarded_lambda([&] {
+        if (!t || !grad) throw Error(CWA_EINVAL, "null argument");
+        double g[GRAD_NUM];
+        t->impl->Backward(dheight0, dheight1, dpos, dvel, g, dpos0, dvel0);
+        grad->visc = g[GRAD_VISC];
+        grad->gas_const = g[GRAD_GAS_CONST];
+        grad->rest_rho = g[GRAD_REST_RHO];
+        grad->lambda_x = g[GRAD_LAMBDA_X];
+        grad->lambda_z = g[GRAD_LAMBDA_Z];
+        grad->atten = g[GRAD_ATTEN];
+        grad->couple_strength = g[GRAD_KAPPA];
+        grad->beta = g[GRAD_BETA];
+    });
+}
+
+int cwa_tape_destroy(cwa_tape* t) {
+    return guarded_lambda([&] {
+        if (!t) return;
+        delete t->impl;
+        delete t;
+    });
+}
+
+}  // extern "C"

@@ -175,7 +175,7 @@ typedef struct cwa_scene cwa_scene;
 int cwa_abi_version(void);
 /* sizeof of the ABI structs, for binding checks: 0 cwa_params, 1 cwa_init,
  * 2 cwa_particle_view, 3 cwa_grid_view, 4 cwa_host_state,
- * 5 cwa_kernel_times; -1 for an unknown index. */
+ * 5 cwa_kernel_times, 6 cwa_grad; -1 for an unknown index. */
 int cwa_struct_size(int which);
 /* Number of visible HIP devices (0 when none), or a negative status. */
 int cwa_device_count(void);
@@ -247,6 +247,43 @@ int cwa_scan_exclusive_i32(const int32_t* d_in, int32_t* d_out, int n, void* str
 int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float cell_size,
                    const int32_t dims[3], int32_t* d_key, int32_t* d_count,
                    int32_t* d_offset, int32_t* d_list, void* stream);
+
+/* ---- gradients: checkpointed forward + adjoint (no reference
+ *      counterpart; the differentiable form of the step above) ----
+ * The adjoint differentiates the step as run, with every discrete decision
+ * frozen at its forward value (cell sort, neighbour sets, EOS / rest
+ * density clamps, coupling band, foam, surface and box clamps, crest).
+ * Requires torque_coeff == 0.  Parameter gradients are returned for the
+ * fields below; lambda = (c dt / dx)^2 gives d/dc by the chain rule. */
+typedef struct cwa_grad {
+    double visc;             /* d loss / d cwa_params.visc */
+    double gas_const;        /* .gas_const */
+    double rest_rho;         /* .rest_rho */
+    double lambda_x;         /* .lambda_x */
+    double lambda_z;         /* .lambda_z */
+    double atten;            /* .atten (the wave damping) */
+    double couple_strength;  /* .couple_strength */
+    double beta;             /* .beta */
+} cwa_grad;
+
+typedef struct cwa_tape cwa_tape;
+
+/* A tape over `nsteps` steps of scene s from its current state, with a
+ * checkpoint every `ckpt_every` steps.  s must outlive the tape. */
+int cwa_tape_create(cwa_scene* s, int nsteps, int ckpt_every, cwa_tape** out);
+/* Run the nsteps (the scene ends in the final state), storing checkpoints. */
+int cwa_tape_forward(cwa_tape* t, float dt);
+/* Adjoint sweep for the loss whose derivatives with respect to the final
+ * state are the seeds (host arrays; NULL = zero): dheight0 / dheight1
+ * (w*h, newest / previous heightfield), dpos / dvel (4*n, original particle
+ * order, xyz used).  Writes the parameter gradients and, if not NULL, the
+ * gradients with respect to the initial positions / velocities (4*n,
+ * original order).  Recomputes each segment from its checkpoint; the scene
+ * is left in the final state. */
+int cwa_tape_backward(cwa_tape* t, const float* dheight0, const float* dheight1,
+                      const float* dpos, const float* dvel, cwa_grad* grad, float* dpos0,
+                      float* dvel0);
+int cwa_tape_destroy(cwa_tape* t);
 
 /* ---- scene generators (host only, no device use) ----
  * Random numbers: SplitMix64 stream from `seed`; u = (x >> 40) * 2^-24.
