@@ -50,7 +50,47 @@ def parse():
                     help="steps of the per-kernel event pass (0: the same K as timed)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="0 disables the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
+    ap.add_argument("--adjoint-steps", type=int, default=256,
+                    help="steps of the forward+adjoint measurement (0 disables it)")
     return ap.parse_args()
+
+
+def measure_adjoint(args) -> dict:
+    """fwd+adjoint / fwd on the differentiable workload: 65 536 particles +
+    512^2 heightfield, checkpoint every 16 steps.  fwd = the same steps
+    launched eagerly without a tape; fwd+adjoint = checkpointed forward +
+    adjoint sweep (segment recompute included) for an MSE height loss."""
+    import numpy as np
+
+    from coupledwateranimation_amd import configs
+    from coupledwateranimation_amd.gradient import Tape
+
+    spec = configs.dam_break_diff(steps=args.adjoint_steps)
+    steps, every = args.adjoint_steps, 16
+    scene = spec.make_scene()
+    scene.step(spec.dt, 2)  # warm-up (kernels loaded, caches warm)
+    scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+    scene.sync()
+    t0 = time.perf_counter()
+    scene.step(spec.dt, steps)
+    scene.sync()
+    t_fwd = time.perf_counter() - t0
+    target = np.zeros((spec.wave_size[1], spec.wave_size[0]), np.float32)
+    scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+    tape = Tape(scene, steps, every)
+    scene.sync()
+    t0 = time.perf_counter()
+    tape.forward(spec.dt)
+    h = scene.readback()["height0"]
+    grad, _, _ = tape.backward(dheight0=2.0 * (h - target) / h.size)
+    t_fa = time.perf_counter() - t0
+    tape.close()
+    scene.close()
+    print(f"bench: adjoint {steps} steps: fwd {1e3 * t_fwd:.2f} ms, fwd+adjoint "
+          f"{1e3 * t_fa:.2f} ms", flush=True)
+    return {"workload": spec.name, "steps": steps, "ckpt_every": every,
+            "fwd_ms": 1e3 * t_fwd, "fwd_adjoint_ms": 1e3 * t_fa, "ratio": t_fa / t_fwd,
+            "finite": bool(np.isfinite(list(grad.values())).all())}
 
 
 def main() -> int:
@@ -125,6 +165,8 @@ def main() -> int:
         except (ValueError, AttributeError):
             traffic = None
 
+    adjoint = measure_adjoint(args) if args.adjoint_steps > 0 else None
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import oracle as O
@@ -173,6 +215,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "finite": finite,
         "state_after": state_stats,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -71,10 +71,12 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Block-reduce NP partial sums and add them to the fp64 gradient slots.
+// Block-reduce NP partial sums and add them to this block's own fp64 slots
+// acc[block * NP + k]; each block owns its slots, so steps accumulate in
+// stream order without atomics and the final sum over blocks runs in a
+// fixed order (k_reduce_partials).
 template <int NP>
-__device__ __forceinline__ void block_add(float (&v)[NP], const int (&slot)[NP],
-                                          double* __restrict__ pgrad) {
+__device__ __forceinline__ void block_accumulate(float (&v)[NP], double* __restrict__ acc) {
     __shared__ float red[NP][4];  // blocks of 256 threads: 4 waves
     const int nw = (blockDim.x * blockDim.y + 63) >> 6;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -88,7 +90,8 @@ __device__ __forceinline__ void block_add(float (&v)[NP], const int (&slot)[NP],
     if (tid < NP) {
         float t = 0.0f;
         for (int q = 0; q < nw; ++q) t += red[tid][q];
-        atomicAdd(&pgrad[slot[tid]], (double)t);
+        const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        acc[b * NP + tid] += (double)t;
     }
 }
 
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
                                                   const float* __restrict__ G1,
                                                   float* __restrict__ G0n,
                                                   float* __restrict__ G1n,
-                                                  double* __restrict__ pgrad) {
+                                                  double* __restrict__ acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = P.W, H = P.H;
@@ -191,8 +194,7 @@ __global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
             part[4] = g * dt * S * P.inv_area;          // kappa
         }
     }
-    const int slot[5] = {GRAD_LAMBDA_X, GRAD_LAMBDA_Z, GRAD_ATTEN, GRAD_BETA, GRAD_KAPPA};
-    block_add<5>(part, slot, pgrad);
+    block_accumulate<5>(part, acc);  // lambda_x, lambda_z, atten, beta, kappa
 }
 
 // ---------------------------------------------------------------------------
@@ -204,9 +206,11 @@ __global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
 __global__ void __launch_bounds__(256) k_adj_colflux(DevParams P, float dt,
                                                      const float* __restrict__ G0,
                                                      float* __restrict__ Sbar) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave64 per column; lanes stride over the column's texel window
+    const int col = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     const int nx = P.dims[0], nz = P.dims[2];
-    if (col >= nx * nz) return;
+    if (col >= nx * nz) return;  // uniform across the wave
     const int cx = col % nx, cz = col / nx;  // Sbar / colflux layout [iz*nx + ix]
     const int W = P.W, H = P.H;
     // texel range whose gx lies in (cx - 1, cx + 1), widened by one texel;
@@ -216,26 +220,41 @@ __global__ void __launch_bounds__(256) k_adj_colflux(DevParams P, float dt,
     if (cx < nx - 1) ihi = min(W - 1, (int)ceilf(((float)cx + 1.0f - P.bx) / P.ax) + 1);
     if (cz > 0) jlo = max(0, (int)floorf(((float)cz - 1.0f - P.bz) / P.az) - 1);
     if (cz < nz - 1) jhi = min(H - 1, (int)ceilf(((float)cz + 1.0f - P.bz) / P.az) + 1);
+    const int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
+    const long long total = (ni > 0 && nj > 0) ? (long long)ni * nj : 0;
     float acc = 0.0f;
-    for (int j = jlo; j <= jhi; ++j) {
+    for (long long t = lane; t < total; t += 64) {
+        const int j = jlo + (int)(t / ni), i = ilo + (int)(t % ni);
         const float gz = P.az * (float)j + P.bz;
         float fz = floorf(gz);
         const float bz = gz - fz;
         fz = fminf(fmaxf(fz, -2.0f), (float)(nz + 1));
         const int z0 = min(max((int)fz, 0), nz - 1), z1 = min(max((int)fz + 1, 0), nz - 1);
         const float wz = (z0 == cz ? 1.0f - bz : 0.0f) + (z1 == cz ? bz : 0.0f);
-        if (wz == 0.0f) continue;
-        for (int i = ilo; i <= ihi; ++i) {
-            const float gx = P.ax * (float)i + P.bx;
-            float fx = floorf(gx);
-            const float bx = gx - fx;
-            fx = fminf(fmaxf(fx, -2.0f), (float)(nx + 1));
-            const int x0 = min(max((int)fx, 0), nx - 1), x1 = min(max((int)fx + 1, 0), nx - 1);
-            const float wx = (x0 == cx ? 1.0f - bx : 0.0f) + (x1 == cx ? bx : 0.0f);
-            if (wx != 0.0f) acc += G0[(size_t)j * W + i] * (wx * wz);
-        }
+        const float gx = P.ax * (float)i + P.bx;
+        float fx = floorf(gx);
+        const float bx = gx - fx;
+        fx = fminf(fmaxf(fx, -2.0f), (float)(nx + 1));
+        const int x0 = min(max((int)fx, 0), nx - 1), x1 = min(max((int)fx + 1, 0), nx - 1);
+        const float wx = (x0 == cx ? 1.0f - bx : 0.0f) + (x1 == cx ? bx : 0.0f);
+        acc += G0[(size_t)j * W + i] * (wx * wz);
     }
-    Sbar[col] = P.kappa * dt * P.inv_area * acc;
+    acc = wave_sum(acc);
+    if (lane == 0) Sbar[col] = P.kappa * dt * P.inv_area * acc;
+}
+
+// Sum the per-block parameter partials (np slots per block, nb blocks) in
+// block order and add them to the gradient slots slot[0..np).
+struct SlotMap {
+    int s[8];
+};
+__global__ void k_reduce_partials(const double* __restrict__ acc, int nb, int np, SlotMap map,
+                                  double* __restrict__ pgrad) {
+    const int k = threadIdx.x;
+    if (k >= np) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += acc[(size_t)b * np + k];
+    pgrad[map.s[k]] += t;
 }
 
 // ---------------------------------------------------------------------------
@@ -370,7 +389,7 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
     const float* __restrict__ rho_raw, const float* __restrict__ H0,
     const float* __restrict__ Sbar, const float4* __restrict__ adjrec,
     const float* __restrict__ rhob_d, float* __restrict__ G0n, float4* __restrict__ xb_loc,
-    float4* __restrict__ vb_loc, float* __restrict__ rtb, double* __restrict__ pgrad) {
+    float4* __restrict__ vb_loc, float* __restrict__ rtb, double* __restrict__ acc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float part[3] = {0.f, 0.f, 0.f};  // visc, gas_const, rest_rho
     if (k < n) {
@@ -487,8 +506,7 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
         xb_loc[k] = make_float4(xl.x + xbx, xl.y + xby, xl.z + xbz, 0.0f);
         vb_loc[k] = make_float4(vl.x + vbx, vl.y + vby, vl.z + vbz, 0.0f);
     }
-    const int slot[3] = {GRAD_VISC, GRAD_GAS_CONST, GRAD_REST_RHO};
-    block_add<3>(part, slot, pgrad);
+    block_accumulate<3>(part, acc);  // visc, gas_const, rest_rho
 }
 
 // ---------------------------------------------------------------------------
@@ -528,18 +546,29 @@ __global__ void __launch_bounds__(256) k_adj_rho_pairs(
 // ---------------------------------------------------------------------------
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
+int adj_wave_blocks(const DevParams& P) { return ((P.W + 63) / 64) * ((P.H + 3) / 4); }
+int adj_pair_blocks(int n) { return nblk(n, 256); }
+
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,
                            const float* colflux, const float* G0, const float* G1, float* G0n,
-                           float* G1n, double* pgrad, hipStream_t st) {
+                           float* G1n, double* acc, hipStream_t st) {
     dim3 blk(64, 4), grd((P.W + 63) / 64, (P.H + 3) / 4);
-    k_adj_wave<<<grd, blk, 0, st>>>(P, dt, H0, H1, colflux, G0, G1, G0n, G1n, pgrad);
+    k_adj_wave<<<grd, blk, 0, st>>>(P, dt, H0, H1, colflux, G0, G1, G0n, G1n, acc);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_colflux(const DevParams& P, float dt, const float* G0, float* Sbar,
                               hipStream_t st) {
-    const int ncol = P.dims[0] * P.dims[2];
-    k_adj_colflux<<<nblk(ncol, 256), 256, 0, st>>>(P, dt, G0, Sbar);
+    const long long ncol = (long long)P.dims[0] * P.dims[2];
+    k_adj_colflux<<<nblk(ncol * 64, 256), 256, 0, st>>>(P, dt, G0, Sbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double* acc, int nb, int np, const int* slots,
+                                  double* pgrad, hipStream_t st) {
+    SlotMap m{};
+    for (int k = 0; k < np && k < 8; ++k) m.s[k] = slots[k];
+    k_reduce_partials<<<1, 64, 0, st>>>(acc, nb, np, m, pgrad);
     return hipGetLastError();
 }
 
@@ -559,12 +588,12 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             const float4* nrec, const float* rho_w, const float2* pw,
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
-                            float4* xb_loc, float4* vb_loc, float* rtb, double* pgrad,
+                            float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P, dt, n, pos, vel, offset, nbmask, nrec, rho_w,
                                               pw, rho_raw, H0, Sbar, adjrec, rhob_d, G0n,
-                                              xb_loc, vb_loc, rtb, pgrad);
+                                              xb_loc, vb_loc, rtb, acc);
     return hipGetLastError();
 }
 

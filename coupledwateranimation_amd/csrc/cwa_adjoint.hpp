@@ -20,9 +20,15 @@ enum GradSlot {
     GRAD_NUM
 };
 
+// Parameter partials are accumulated per block (fp64, no atomics) into acc
+// arrays of blocks x slots; launch_reduce_partials sums them in block order.
+int adj_wave_blocks(const DevParams& P);  // blocks of launch_adj_wave (5 slots each)
+int adj_pair_blocks(int n);               // blocks of launch_adj_pairs (3 slots each)
+hipError_t launch_reduce_partials(const double* acc, int nb, int np, const int* slots,
+                                  double* pgrad, hipStream_t st);
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,
                            const float* colflux, const float* G0, const float* G1, float* G0n,
-                           float* G1n, double* pgrad, hipStream_t st);
+                           float* G1n, double* acc, hipStream_t st);
 hipError_t launch_adj_colflux(const DevParams& P, float dt, const float* G0, float* Sbar,
                               hipStream_t st);
 hipError_t launch_adj_local(const DevParams& P, float dt, int n, const float4* pos,
@@ -35,7 +41,7 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             const float4* nrec, const float* rho_w, const float2* pw,
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
-                            float4* xb_loc, float4* vb_loc, float* rtb, double* pgrad,
+                            float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st);
 hipError_t launch_adj_rho_pairs(const DevParams& P, int n, const float4* pos, const int* offset,
                                 const uint2* nbmask, const float* rtb, const float4* xb_loc,

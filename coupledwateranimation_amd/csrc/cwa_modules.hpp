@@ -121,8 +121,10 @@ class UniformGrid3D {
    public:
     void Init(const GridDesc& g, int max_particles);
     // BuildGrid (UniformGridParticles3D.cpp:170-211) over pos[0..n).
+    // list_out / offset_out (optional) replace the grid's own list / offsets.
     void BuildGrid(const float4* pos, int n, const ParticleArrays* in, ParticleArrays* out,
-                   hipStream_t st, hipEvent_t* ev = nullptr);
+                   hipStream_t st, hipEvent_t* ev = nullptr, int* list_out = nullptr,
+                   int* offset_out = nullptr);
     const GridDesc& desc() const { return g_; }
     int num_cells() const { return g_.dims[0] * g_.dims[1] * g_.dims[2]; }
     int* key() const { return key_.get(); }
@@ -200,10 +202,21 @@ class SphParticles3D {
     DeviceBuffer<float2> pw_;
 };
 
-// Optional per-step outputs recorded for the adjoint tape.
+// Optional per-step output locations used by the adjoint tape: when set,
+// the step writes these arrays there instead of into the scene's own
+// buffers (and reads them back from there within the step).
 struct TapeOut {
     float* rho_raw = nullptr;  // density before the wave term and the clamp
     float4* fpre = nullptr;    // force before the foam branch
+    float4* pos = nullptr;     // sorted state at the start of the step
+    float4* vel = nullptr;
+    int* list = nullptr;       // stable cell order of the step
+    int* offset = nullptr;     // cell offsets of the step
+    uint2* nbmask = nullptr;   // neighbour masks
+    float4* nrec = nullptr;    // neighbour records
+    float* rho = nullptr;      // density (after the clamp)
+    float2* pw = nullptr;      // (pressure, 1/rho)
+    float* colflux = nullptr;  // column flux
 };
 
 // Kernel slots of cwa_kernel_times.

@@ -78,8 +78,11 @@ void UniformGrid3D::ClearCounters(hipStream_t st) {
 }
 
 void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in,
-                              ParticleArrays* out, hipStream_t st, hipEvent_t* ev) {
+                              ParticleArrays* out, hipStream_t st, hipEvent_t* ev,
+                              int* list_out, int* offset_out) {
     const int nc = num_cells();
+    int* lst = list_out ? list_out : list_.get();
+    int* off = offset_out ? offset_out : offset_.get();
     auto mark = [&](int slot, int which) {
         if (ev) hip_check(hipEventRecord(ev[2 * slot + which], st), "event");
     };
@@ -88,17 +91,15 @@ void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in
     hip_check(launch_hash_count(pos, n, g_, key_.get(), trank_.get(), cnt, st), "hash");
     mark(K_HASH, 1);
     mark(K_SCAN, 0);
-    hip_check(launch_scan(cnt, offset_.get(), nc, ws_.get(), st, count_[cur_ ^ 1].get()), "scan");
+    hip_check(launch_scan(cnt, off, nc, ws_.get(), st, count_[cur_ ^ 1].get()), "scan");
     mark(K_SCAN, 1);
     last_ = cur_;
     cur_ ^= 1;
     mark(K_SCATTER, 0);
-    hip_check(launch_scatter_tmp(key_.get(), trank_.get(), offset_.get(), n, tmp_.get(), st),
-              "scatter");
+    hip_check(launch_scatter_tmp(key_.get(), trank_.get(), off, n, tmp_.get(), st), "scatter");
     mark(K_SCATTER, 1);
     mark(K_REORDER, 0);
-    hip_check(launch_stable_reorder(key_.get(), offset_.get(), tmp_.get(), n, list_.get(), in,
-                                    out, st),
+    hip_check(launch_stable_reorder(key_.get(), off, tmp_.get(), n, lst, in, out, st),
               "reorder");
     mark(K_REORDER, 1);
 }
@@ -410,32 +411,41 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
     const int n = particles.n();
     const float* H0 = wave.GetReadImage(0);
     const bool couple = n > 0 && params_.couple_band > 0.0f;
+    // where this step's arrays live (the scene's own, or the tape's)
+    auto pick = [&](auto* t, auto* own) { return (tape && t) ? t : own; };
+    ParticleArrays B = particles.B;
+    B.pos = pick(tape ? tape->pos : nullptr, particles.B.pos);
+    B.vel = pick(tape ? tape->vel : nullptr, particles.B.vel);
+    int* off = pick(tape ? tape->offset : nullptr, grid.offset());
+    int* lst = pick(tape ? tape->list : nullptr, grid.list());
+    uint2* nbm = pick(tape ? tape->nbmask : nullptr, particles.nbmask());
+    float4* nrec = pick(tape ? tape->nrec : nullptr, particles.nrec());
+    float* rho_w = pick(tape ? tape->rho : nullptr, particles.rho_w());
+    float2* pw = pick(tape ? tape->pw : nullptr, particles.pw());
+    float* cflux = pick(tape ? tape->colflux : nullptr, colflux.get());
     if (n > 0) {
-        grid.BuildGrid(particles.A.pos, n, &particles.A, &particles.B, st_,
-                       ev ? &ev[0][0] : nullptr);
+        grid.BuildGrid(particles.A.pos, n, &particles.A, &B, st_, ev ? &ev[0][0] : nullptr, lst,
+                       off);
         mark(K_DENSITY, 0);
-        hip_check(launch_density(dp_, particles.B, grid.offset(), n, H0, particles.rho_w(),
-                                 particles.pw(), particles.flux(), particles.nbmask(),
-                                 particles.nrec(), tape ? tape->rho_raw : nullptr, st_),
+        hip_check(launch_density(dp_, B, off, n, H0, rho_w, pw, particles.flux(), nbm, nrec,
+                                 tape ? tape->rho_raw : nullptr, st_),
                   "density");
         mark(K_DENSITY, 1);
         mark(K_FORCE, 0);
-        hip_check(launch_force_integrate(dp_, dt, particles.B, grid.offset(), n, particles.rho_w(),
-                                         particles.pw(), particles.nbmask(), particles.nrec(), H0,
+        hip_check(launch_force_integrate(dp_, dt, B, off, n, rho_w, pw, nbm, nrec, H0,
                                          particles.A, particles.rho(), particles.prs(),
                                          tape ? tape->fpre : nullptr, st_),
                   "force/integrate");
         mark(K_FORCE, 1);
         if (couple) {
             mark(K_COLFLUX, 0);
-            hip_check(launch_column_flux(grid.offset(), particles.flux(), params_.grid_dims,
-                                         colflux.get(), st_),
+            hip_check(launch_column_flux(off, particles.flux(), params_.grid_dims, cflux, st_),
                       "column flux");
             mark(K_COLFLUX, 1);
         }
     }
     mark(K_WAVE, 0);
-    wave.Evolve(dp_, couple ? colflux.get() : nullptr, dt, st_);
+    wave.Evolve(dp_, couple ? cflux : nullptr, dt, st_);
     mark(K_WAVE, 1);
     wave.PingPong();
     if (params_.check_finite && !ev) CheckFinite();

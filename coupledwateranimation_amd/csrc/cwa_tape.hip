@@ -31,7 +31,7 @@ class Tape {
     struct Ckpt {
         DeviceBuffer<float4> pos, vel, force;
         DeviceBuffer<int> id;
-        DeviceBuffer<float> h0, h1;
+        DeviceBuffer<float> rho, prs, h0, h1;
     };
     struct StepTape {
         DeviceBuffer<float4> pos, vel, nrec, fpre;
@@ -61,7 +61,7 @@ class Tape {
     std::vector<StepTape> tp_;
     DeviceBuffer<float4> xb_, vb_, xb2_, vb2_, xl_, vl_, adjrec_;
     DeviceBuffer<float> G0_, G1_, G0n_, G1n_, rhob_, rtb_, Sbar_;
-    DeviceBuffer<double> pgrad_;
+    DeviceBuffer<double> pgrad_, acc_wave_, acc_pair_;
 };
 
 Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
@@ -84,6 +84,8 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
         c.vel.resize(n);
         c.force.resize(n);
         c.id.resize(n);
+        c.rho.resize(n);
+        c.prs.resize(n);
         c.h0.resize(tex);
         c.h1.resize(tex);
     }
@@ -110,6 +112,8 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
     rtb_.resize(n);
     Sbar_.resize(ncol_);
     pgrad_.resize(GRAD_NUM);
+    acc_wave_.resize((size_t)adj_wave_blocks(s.devparams()) * 5);
+    acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
 }
 
 void Tape::save(Ckpt& c) {
@@ -118,6 +122,8 @@ void Tape::save(Ckpt& c) {
     copy(c.vel.get(), A.vel, n_);
     copy(c.force.get(), A.force, n_);
     copy(c.id.get(), A.id, n_);
+    copy(c.rho.get(), S_.particles.rho(), n_);
+    copy(c.prs.get(), S_.particles.prs(), n_);
     copy(c.h0.get(), S_.wave.GetReadImage(0), (size_t)W_ * H_);
     copy(c.h1.get(), S_.wave.GetReadImage(1), (size_t)W_ * H_);
 }
@@ -128,6 +134,8 @@ void Tape::restore(const Ckpt& c) {
     copy(A.vel, c.vel.get(), n_);
     copy(A.force, c.force.get(), n_);
     copy(A.id, c.id.get(), n_);
+    copy(S_.particles.rho(), c.rho.get(), n_);
+    copy(S_.particles.prs(), c.prs.get(), n_);
     copy(S_.wave.GetReadImage(0), c.h0.get(), (size_t)W_ * H_);
     copy(S_.wave.GetReadImage(1), c.h1.get(), (size_t)W_ * H_);
     S_.grid.ClearCounters(st_);
@@ -147,7 +155,9 @@ void Tape::Forward(float dt) {
     forward_done_ = true;
 }
 
-// One step of the segment replay, recording what the adjoint reads.
+// One step of the segment replay, recording what the adjoint reads: the
+// step writes its intermediates straight into the tape slot; only the two
+// heightfields it reads are copied (the triple buffer reuses them).
 void Tape::record_step(StepTape& t) {
     const size_t tex = (size_t)W_ * H_;
     copy(t.h0.get(), S_.wave.GetReadImage(0), tex);
@@ -155,24 +165,23 @@ void Tape::record_step(StepTape& t) {
     TapeOut out;
     out.rho_raw = t.rho_raw.get();
     out.fpre = t.fpre.get();
+    out.pos = t.pos.get();
+    out.vel = t.vel.get();
+    out.list = t.list.get();
+    out.offset = t.offset.get();
+    out.nbmask = t.nbmask.get();
+    out.nrec = t.nrec.get();
+    out.rho = t.rho.get();
+    out.pw = t.pw.get();
+    out.colflux = t.colflux.get();
     S_.Step(dt_, nullptr, &out);
-    auto& P = S_.particles;
-    copy(t.pos.get(), P.B.pos, n_);
-    copy(t.vel.get(), P.B.vel, n_);
-    copy(t.nrec.get(), P.nrec(), 2 * (size_t)n_);
-    copy(t.list.get(), S_.grid.list(), n_);
-    copy(t.offset.get(), S_.grid.offset(), (size_t)ncell_ + 1);
-    copy(t.nbmask.get(), P.nbmask(), 9 * (size_t)n_);
-    copy(t.rho.get(), P.rho_w(), n_);
-    copy(t.pw.get(), P.pw(), n_);
-    copy(t.colflux.get(), S_.colflux.get(), ncol_);
 }
 
 void Tape::adjoint_step(const StepTape& t) {
     const DevParams& P = S_.devparams();
     const bool couple = n_ > 0 && P.band > 0.0f;
     hip_check(launch_adj_wave(P, dt_, t.h0.get(), t.h1.get(), couple ? t.colflux.get() : nullptr,
-                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), pgrad_.get(), st_),
+                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(), st_),
               "adj wave");
     if (n_ > 0) {
         if (couple)
@@ -185,7 +194,7 @@ void Tape::adjoint_step(const StepTape& t) {
                                    t.nbmask.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
                                    t.rho_raw.get(), t.h0.get(), Sbar_.get(), adjrec_.get(),
                                    rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
-                                   pgrad_.get(), st_),
+                                   acc_pair_.get(), st_),
                   "adj pairs");
         hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbmask.get(),
                                        rtb_.get(), xl_.get(), vl_.get(), t.list.get(),
@@ -231,6 +240,8 @@ void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const
     upload_field(dH0, G0_);
     upload_field(dH1, G1_);
     hip_check(hipMemsetAsync(pgrad_.get(), 0, GRAD_NUM * sizeof(double), st_), "grad zero");
+    hip_check(hipMemsetAsync(acc_wave_.get(), 0, acc_wave_.size() * sizeof(double), st_), "acc");
+    hip_check(hipMemsetAsync(acc_pair_.get(), 0, acc_pair_.size() * sizeof(double), st_), "acc");
 
     for (int seg = nseg_ - 1; seg >= 0; --seg) {
         restore(ck_[seg]);
@@ -238,6 +249,14 @@ void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const
         for (int k = 0; k < len; ++k) record_step(tp_[k]);
         for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k]);
     }
+    const int wslots[5] = {GRAD_LAMBDA_X, GRAD_LAMBDA_Z, GRAD_ATTEN, GRAD_BETA, GRAD_KAPPA};
+    const int pslots[3] = {GRAD_VISC, GRAD_GAS_CONST, GRAD_REST_RHO};
+    hip_check(launch_reduce_partials(acc_wave_.get(), (int)(acc_wave_.size() / 5), 5, wslots,
+                                     pgrad_.get(), st_),
+              "reduce");
+    hip_check(launch_reduce_partials(acc_pair_.get(), (int)(acc_pair_.size() / 3), 3, pslots,
+                                     pgrad_.get(), st_),
+              "reduce");
     restore(ck_[nseg_]);  // leave the scene in its final state
     hip_check(hipStreamSynchronize(st_), "backward");
 
