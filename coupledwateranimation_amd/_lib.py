@@ -103,6 +103,60 @@ class Grad(C.Structure):
 
 STRUCTS = [Params, Init, ParticleView, GridView, HostState, KernelTimes, Grad]
 
+# --- batched 2D step (include/cwa2d.h) -------------------------------------
+CWA2D_NUM_KERNELS = 7
+
+
+class Params2D(C.Structure):
+    """cwa2d_params (include/cwa2d.h)."""
+
+    _fields_ = [
+        ("h", C.c_float), ("rest_rho", C.c_float), ("gas_const", C.c_float),
+        ("visc", C.c_float), ("psi", C.c_float), ("gravity", f2), ("lo", f2), ("hi", f2),
+        ("walls", C.c_float * 4), ("enable_vel", C.c_float), ("wave_ref", C.c_float),
+        ("wave_press", C.c_float), ("wave_dy", C.c_float), ("wave_maskv", C.c_float),
+        ("atten", C.c_float), ("beta", C.c_float), ("couple_band", C.c_float),
+        ("couple_strength", C.c_float), ("sph_substeps", C.c_int32),
+        ("check_finite", C.c_int32),
+    ]
+
+    def copy(self) -> "Params2D":
+        p = Params2D()
+        C.pointer(p)[0] = self
+        return p
+
+
+class Scene2D(C.Structure):
+    _fields_ = [("n", C.c_int32), ("wave_len", C.c_int32), ("wave_substeps", C.c_int32),
+                ("mass", C.c_float), ("lambda_", C.c_float)]
+
+
+class Tuning2D(C.Structure):
+    _fields_ = [("block_density", C.c_int32), ("block_force", C.c_int32),
+                ("grid_density", C.c_int32), ("grid_force", C.c_int32),
+                ("xcd_queues", C.c_int32), ("wave_block", C.c_int32),
+                ("ms_per_step", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Views2D(C.Structure):
+    _fields_ = [
+        ("nscenes", C.c_int32), ("n_total", C.c_int32), ("wave_total", C.c_int32),
+        ("grid_nx", C.c_int32), ("grid_ny", C.c_int32), ("cells_total", C.c_int32),
+        ("xv", C.c_void_p), ("rp", C.c_void_p), ("idf", C.c_void_p), ("key", C.c_void_p),
+        ("offset", C.c_void_p), ("h0", C.c_void_p), ("h1", C.c_void_p),
+        ("scene_begin", C.c_void_p), ("wave_begin", C.c_void_p),
+    ]
+
+
+class KernelTimes2D(C.Structure):
+    _fields_ = [("ms", C.c_float * CWA2D_NUM_KERNELS)]
+
+
+STRUCTS_2D = [Params2D, Scene2D, Tuning2D, Views2D, KernelTimes2D]
+
 # exported symbol -> (restype, argtypes)
 SIGNATURES = {
     "cwa_abi_version": (C.c_int, []),
@@ -146,6 +200,34 @@ SIGNATURES = {
                                   C.POINTER(C.c_int32), C.c_uint64]),
     "cwa_gen_bumps": (C.c_int, [C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_uint64]),
+    # batched 2D (include/cwa2d.h)
+    "cwa2d_abi_version": (C.c_int, []),
+    "cwa2d_struct_size": (C.c_int, [C.c_int]),
+    "cwa2d_kernel_name": (C.c_char_p, [C.c_int]),
+    "cwa2d_params_reference": (C.c_int, [C.POINTER(Params2D)]),
+    "cwa2d_wave_init_reference": (C.c_int, [C.c_int32, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float)]),
+    "cwa2d_batch_create": (C.c_int, [C.POINTER(Params2D), C.c_int32, C.POINTER(Scene2D),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_void_p)]),
+    "cwa2d_batch_destroy": (C.c_int, [C.c_void_p]),
+    "cwa2d_batch_reinit": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cwa2d_batch_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params2D)]),
+    "cwa2d_batch_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params2D)]),
+    "cwa2d_batch_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int32]),
+    "cwa2d_batch_step_graph": (C.c_int, [C.c_void_p, C.c_float, C.c_int32]),
+    "cwa2d_batch_step_profiled": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(KernelTimes2D)]),
+    "cwa2d_batch_sync": (C.c_int, [C.c_void_p]),
+    "cwa2d_batch_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "cwa2d_batch_views": (C.c_int, [C.c_void_p, C.POINTER(Views2D)]),
+    "cwa2d_batch_readback": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]),
+    "cwa2d_batch_tune": (C.c_int, [C.c_void_p, C.c_float, C.c_int32, C.POINTER(Tuning2D)]),
+    "cwa2d_batch_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning2D)]),
+    "cwa2d_batch_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning2D)]),
 }
 
 _lib = None
@@ -183,6 +265,11 @@ def lib() -> C.CDLL:
         for k, st in enumerate(STRUCTS):
             if L.cwa_struct_size(k) != C.sizeof(st):
                 raise RuntimeError(f"struct {st.__name__} size mismatch with libcwa")
+        if L.cwa2d_abi_version() != 1:
+            raise RuntimeError("libcwa 2D ABI version mismatch")
+        for k, st in enumerate(STRUCTS_2D):
+            if L.cwa2d_struct_size(k) != C.sizeof(st):
+                raise RuntimeError(f"struct {st.__name__} size mismatch with libcwa")
         _lib = L
     return _lib
 
@@ -190,6 +277,13 @@ def lib() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != CWA_OK:
         raise CwaError(rc, lib().cwa_last_error().decode())
+
+
+def iptr(a):
+    """int32* of a C-contiguous int32 numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
 def fptr(a):

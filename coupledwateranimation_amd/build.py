@@ -19,7 +19,11 @@ OUT = PKG / "_build"
 LIB = OUT / "libcwa.so"
 ROOT = PKG.parent
 
-SOURCES = ["cwa_kernels.hip", "cwa_adjoint.hip", "cwa_scene.hip", "cwa_tape.hip", "cwa_gen.cpp"]
+SOURCES = ["cwa_kernels.hip", "cwa_adjoint.hip", "cwa_scene.hip", "cwa_tape.hip", "cwa_batch2d.hip",
+           "cwa_gen.cpp"]
+# The batched 2D step reproduces the oracle's float expressions bit for bit:
+# no contraction into fused multiply-adds beyond the explicit fmaf calls.
+EXTRA = {"cwa_batch2d.hip": ["-ffp-contract=off"]}
 HEADERS = ["cwa_device.hpp", "cwa_kernels.hpp", "cwa_adjoint.hpp", "cwa_modules.hpp"]
 
 ARCH = os.environ.get("CWA_OFFLOAD_ARCH", "gfx950")
@@ -52,6 +56,7 @@ def build(verbose: bool = True) -> Path:
         cmd = [cc, *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".hip"):
             cmd[1:1] = ["-x", "hip", f"--offload-arch={ARCH}", *DEVICE_FLAGS]
+        cmd[1:1] = EXTRA.get(src, [])
         _run(cmd)
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")

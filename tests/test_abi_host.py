@@ -12,18 +12,56 @@ import pytest
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def _declared_functions():
-    text = (ROOT / "include" / "cwa.h").read_text()
+def _declared_functions(header="cwa.h", prefix="cwa_"):
+    text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(cwa_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text)))
 
 
 def test_exports_every_declared_symbol(cwa):
     L = cwa.lib()
     names = _declared_functions()
     assert len(names) >= 25
-    missing = [n for n in names if not hasattr(L, n)]
+    names2d = _declared_functions("cwa2d.h", "cwa2d_")
+    assert len(names2d) >= 20
+    missing = [n for n in names + names2d if not hasattr(L, n)]
     assert not missing, missing
+    from coupledwateranimation_amd import _lib
+    assert not [n for n in names + names2d if n not in _lib.SIGNATURES]
+
+
+def test_2d_struct_sizes_and_rejections(cwa):
+    """cwa2d structs match the binding; invalid batches are rejected before
+    any device work (no GPU needed)."""
+    from coupledwateranimation_amd import _lib, batch2d
+
+    L = cwa.lib()
+    assert L.cwa2d_abi_version() == 1
+    for k, st in enumerate(_lib.STRUCTS_2D):
+        assert L.cwa2d_struct_size(k) == C.sizeof(st)
+    assert L.cwa2d_struct_size(9) == -1
+    assert [L.cwa2d_kernel_name(k).decode() for k in range(7)] == [
+        "hash", "scan", "scatter", "reorder", "density", "force", "wave"]
+    spec = batch2d.small_batch(nscenes=2)
+    bad = []
+    p = spec.params.copy()
+    p.h = 0.0
+    bad.append((p, spec.scenes()))
+    sc = spec.scenes()
+    sc[1].wave_len = 2
+    bad.append((spec.params, sc))
+    sc = spec.scenes()
+    sc[0].n = 0
+    bad.append((spec.params, sc))
+    p = spec.params.copy()
+    p.sph_substeps = 0
+    bad.append((p, spec.scenes()))
+    for prm, scn in bad:
+        h = C.c_void_p()
+        rc = L.cwa2d_batch_create(C.byref(prm), 2, scn, _lib.fptr(spec.xv), None, None, None,
+                                  C.byref(h))
+        assert rc == _lib.CWA_EINVAL and not h.value
+        assert L.cwa_last_error()
 
 
 def test_struct_sizes_and_version(cwa):

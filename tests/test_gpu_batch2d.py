@@ -1,0 +1,210 @@
+"""GPU parity of the batched 2D step (include/cwa2d.h) against the float
+oracle (oracle/cwa_oracle2d.hpp).
+
+Bar: bit-exact, for every field after every tested number of steps —
+cell keys, offsets, slot order, positions, velocities, density, pressure,
+enable flags and both wave levels.  The kernels evaluate the oracle's
+float expressions in the same order with contraction off, so any
+difference is a defect, not rounding.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def o2():
+    import subprocess
+    from pathlib import Path
+
+    from oracle import oracle2d
+
+    if not oracle2d.LIB_PATH.exists():
+        subprocess.run(["make", "-C", str(Path(oracle2d.__file__).parent)], check=True)
+    oracle2d.lib()
+    return oracle2d
+
+
+def _f32_bits(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32)
+
+
+def _assert_slots_equal(g, o, where=""):
+    for k in ("xv", "rp", "h0", "h1"):
+        gb, ob = _f32_bits(g[k]), _f32_bits(o[k])
+        if not np.array_equal(gb, ob):
+            bad = np.argwhere(gb.reshape(gb.shape[0], -1) != ob.reshape(ob.shape[0], -1))
+            i = bad[0]
+            raise AssertionError(f"{where}: {k} differs at {i.tolist()} "
+                                 f"(gpu {np.asarray(g[k]).reshape(gb.shape[0], -1)[i[0]]} "
+                                 f"oracle {np.asarray(o[k]).reshape(ob.shape[0], -1)[i[0]]}); "
+                                 f"{len(bad)} entries")
+    assert np.array_equal(g["idf"], o["idf"]), f"{where}: slot order / flags differ"
+
+
+def _run_both(spec, steps, graph=False, threads=8, o2=None, tuning=None):
+    b = spec.make()
+    if tuning:
+        b.set_tuning(**tuning)
+    b.step(spec.dt, steps, graph=graph)
+    g = b.slots()
+    o = o2.OracleBatch(spec, threads=threads)
+    o.step(spec.dt, steps)
+    return b, g, o
+
+
+def test_one_step_grid_and_state_bit_exact(gpu, cwa, o2):
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=6, seed=7)
+    _say("small batch", spec.n.tolist(), spec.wave_len.tolist())
+    b, g, o = _run_both(spec, 1, o2=o2)
+    _assert_slots_equal(g, o.slots(), "1 step")
+    nc = g["nx"] * g["ny"]
+    pb = spec.pbeg()
+    for s in range(spec.nscenes):
+        og = o.grid(s)
+        sl = slice(int(pb[s]), int(pb[s] + spec.n[s]))
+        assert np.array_equal(g["key"][sl] - s * nc, og["key"]), f"keys of scene {s}"
+        assert np.array_equal(g["offset"][s * nc:(s + 1) * nc + 1] - pb[s], og["offset"]), \
+            f"offsets of scene {s}"
+    b.close()
+
+
+def test_sixty_steps_bit_exact(gpu, cwa, o2):
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=8, seed=3, lo=200, hi=6000)
+    _say("batch", spec.n.tolist())
+    b, g, o = _run_both(spec, 60, o2=o2)
+    _assert_slots_equal(g, o.slots(), "60 steps")
+    st = b.readback()
+    os_ = o.state()
+    assert np.array_equal(_f32_bits(st["xv"]), _f32_bits(os_["xv"]))
+    assert np.array_equal(st["enabled"], os_["enabled"])
+    b.close()
+
+
+def test_reference_scene_bit_exact(gpu, cwa, o2):
+    """The SphWave2D application's scene: 2 SPH substeps, the 0.05 enable
+    threshold, the 5e5 wave pressure; two copies in one batch."""
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.reference_batch(2)
+    b, g, o = _run_both(spec, 30, o2=o2)
+    _assert_slots_equal(g, o.slots(), "reference scene, 30 steps")
+    assert 0 < o.state()["enabled"].sum() < spec.n_total   # the flag is exercised
+    b.close()
+
+
+def test_one_way_and_flag_variants_bit_exact(gpu, cwa, o2):
+    from coupledwateranimation_amd import batch2d
+
+    p = batch2d.unit_square_params(couple=False)
+    p.enable_vel = 0.5
+    spec = batch2d.small_batch(nscenes=5, seed=21, params=p)
+    spec.enabled = (np.arange(spec.n_total) % 3 != 0).astype(np.int32)
+    b, g, o = _run_both(spec, 25, o2=o2)
+    _assert_slots_equal(g, o.slots(), "one-way + flags")
+    b.close()
+
+
+def test_graph_replay_matches_eager(gpu, cwa, o2):
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=7, seed=4)
+    a = spec.make()
+    a.step(spec.dt, 25)
+    b = spec.make()
+    b.step(spec.dt, 10, graph=True)
+    b.step(spec.dt, 5)                       # eager steps between replays
+    b.step(spec.dt, 10, graph=True)
+    _assert_slots_equal(a.slots(), b.slots(), "graph vs eager")
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(block_density=64, block_force=64, grid_density=0, grid_force=0, xcd_queues=0,
+         wave_block=128),
+    dict(block_density=512, block_force=128, grid_density=256, grid_force=2048, xcd_queues=1,
+         wave_block=1024),
+    dict(block_density=1024, block_force=1024, grid_density=3, grid_force=1, xcd_queues=1,
+         wave_block=64),
+])
+def test_results_independent_of_launch_shape(gpu, cwa, o2, tuning):
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=9, seed=8)
+    _say("tuning", tuning)
+    b, g, o = _run_both(spec, 12, o2=o2, tuning=tuning)
+    _assert_slots_equal(g, o.slots(), f"tuning {tuning}")
+    b.close()
+
+
+def test_autotuner_restores_state(gpu, cwa, o2):
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=12, seed=5, lo=1000, hi=20000)
+    b = spec.make()
+    b.step(spec.dt, 3)
+    before = b.slots()
+    t = b.tune(spec.dt, 2)
+    _say("tuned", t)
+    assert t["ms_per_step"] > 0
+    assert t["block_density"] in (64, 128, 256, 512) and t["wave_block"] in (128, 256, 512, 1024)
+    after = b.slots()
+    _assert_slots_equal(after, before, "after tune")
+    b.step(spec.dt, 4, graph=True)
+    o = o2.OracleBatch(spec)
+    o.step(spec.dt, 7)
+    _assert_slots_equal(b.slots(), o.slots(), "tuned run")
+    b.close()
+
+
+def test_c3_batch_three_steps_bit_exact(gpu, cwa, o2):
+    """All 256 scenes of the batched workload (about 4M particles)."""
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.batch_c3()
+    _say("C3", spec.n_total, "particles", spec.wave_total, "wave cells")
+    b, g, o = _run_both(spec, 3, graph=True, threads=16, o2=o2)
+    _assert_slots_equal(g, o.slots(), "C3 3 steps")
+    b.close()
+
+
+def test_c3_batch_invariants_over_500_steps(gpu, cwa):
+    """Size-independent properties at the full workload: finite state,
+    every scene's slots hold a permutation of its ids, keys sorted within
+    each scene and consistent with the offsets, particles inside the
+    (slightly penetrable) box."""
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.batch_c3()
+    spec.params.check_finite = 1
+    b = spec.make()
+    b.step(spec.dt, spec.steps, graph=True)
+    g = b.slots()
+    assert np.isfinite(g["xv"]).all() and np.isfinite(g["h0"]).all()
+    pb = spec.pbeg()
+    nc = g["nx"] * g["ny"]
+    for s in range(spec.nscenes):
+        sl = slice(int(pb[s]), int(pb[s] + spec.n[s]))
+        ids = g["idf"][sl] & 0x3FFFFFFF
+        assert np.array_equal(np.sort(ids), np.arange(spec.n[s])), f"scene {s} ids"
+        off = g["offset"][s * nc:(s + 1) * nc + 1]
+        assert off[0] == pb[s] and off[-1] == pb[s] + spec.n[s]
+        assert np.all(np.diff(off) >= 0)
+    xy = g["xv"][:, :2]
+    assert xy.min() > -0.05 and xy.max() < 1.05
+    rho = g["rp"][:, 0]
+    assert rho.min() >= 1000.0 and rho.max() < 1e5
+    b.close()
