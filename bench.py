@@ -1,16 +1,22 @@
 """Benchmark of the coupled SPH + wave step on one MI355X.
 
-Workload (default): 1 048 576 particles + 2048^2 heightfield dam break,
-the at-scale configuration; one "step" is one full coupled step (grid
-build, density, forces + integration, particle -> wave flux, wave
-EVOLVE).  Inputs are resident in HBM before timing starts; the timed region
-is K steps launched as hipGraph replays, bracketed by device
-synchronisation.
+Headline workload: 1 048 576 particles + 2048^2 heightfield dam break, the
+at-scale configuration; one "step" is one full coupled step (grid build,
+density, forces + integration, particle -> wave flux, wave EVOLVE).
+Inputs are resident in HBM before timing starts; the timed region is K
+steps launched as hipGraph replays, bracketed by device synchronisation,
+repeated from the initial state (value: the median repeat).
 
 Prints one JSON line: particle-updates/s (value), wave cell-updates/s, the
 roofline of the dominant kernel (per-kernel average device time from HIP
-events recorded on the scene's stream), and the CPU oracle (OpenMP) timed
-on a bounded sample of the same workload.
+events recorded on the scene's stream; HBM bytes per launch from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/traffic.json), the
+measured device-copy ceiling, the forward+adjoint / forward ratio, the
+batched 2D form (256 scenes, tuned, graph-replayed) and the CPU oracle
+(OpenMP) timed on bounded samples of the same workloads.
+
+`--write-traffic FETCH_CSV WRITE_CSV` turns two rocprofv3 --pmc counter
+collections of a bench run into profiles/traffic.json.
 """
 from __future__ import annotations
 
@@ -38,6 +44,16 @@ BYTES_PER_PARTICLE = {
 BYTES_PER_CELL = 8          # scan: read count + write offset
 BYTES_PER_WAVE_CELL = 12    # read h^n, h^(n-1); write h^(n+1)
 
+# Batched 2D step (SoA: xv float4, rp float2, idf int, q float4, aj float2).
+BYTES_PER_PARTICLE_2D = {
+    "hash": 24,      # read xv; write key, run record
+    "scatter": 8,    # read key, run record
+    "reorder": 64,   # read key, record, xv, rp, idf; write xv, rp, idf
+    "density": 68,   # read xv, rp, idf; write q, rp, aj, idf, flux
+    "force": 44,     # read q, aj, idf; write xv
+}
+BYTES_PER_WAVE_CELL_2D = 16  # read and write both levels (substeps stay in LDS)
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -52,7 +68,185 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
     ap.add_argument("--adjoint-steps", type=int, default=256,
                     help="steps of the forward+adjoint measurement (0 disables it)")
+    ap.add_argument("--batched-steps", type=int, default=500,
+                    help="steps of the batched 2D measurement (0 disables it)")
+    ap.add_argument("--batched-cpu-steps", type=int, default=1,
+                    help="oracle steps of the batched CPU baseline (0 disables it)")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed repeats of the K steps (value: the median)")
+    ap.add_argument("--copy-mib", type=int, default=2048,
+                    help="size of the device copy that measures the HBM ceiling (0: skip)")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="derive profiles/traffic.json from two rocprofv3 --pmc counter "
+                         "collections (FETCH_SIZE, WRITE_SIZE) of a bench run, then exit")
     return ap.parse_args()
+
+
+# rocprofv3 kernel name prefix -> (workload key, kernel key of kernel_ms)
+KERNEL_KEYS = [
+    ("cwa::b2d::k2_density<", "batched", "density"), ("cwa::b2d::k2_force<", "batched", "force"),
+    ("cwa::b2d::k2_hash(", "batched", "hash"), ("cwa::b2d::k2_scatter(", "batched", "scatter"),
+    ("cwa::b2d::k2_reorder(", "batched", "reorder"), ("cwa::b2d::k2_wave(", "batched", "wave"),
+    ("cwa::k_density(", "plain", "density"), ("cwa::k_force_integrate(", "plain", "force_integrate"),
+    ("cwa::k_hash_count(", "plain", "hash"), ("cwa::k_scatter_tmp(", "plain", "scatter"),
+    ("cwa::k_stable_reorder(", "plain", "reorder"), ("cwa::k_wave4(", "plain", "wave"),
+    ("cwa::k_column_flux(", "plain", "column_flux"),
+]
+
+
+def write_traffic(args) -> int:
+    """HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by what the same counter reads for the calibration
+    copy (measure_copy: `--copy-mib` MiB read and written), averaged over
+    every dispatch of each kernel."""
+    import csv
+    from collections import defaultdict
+
+    def load(path, counter):
+        per = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                per[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return per
+
+    fetch_csv, write_csv = args.write_traffic
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_kib = args.copy_mib * 1024.0
+    # the calibration copy: the dispatch with the largest FETCH_SIZE
+    cal = max(fetch, key=lambda k: max(fetch[k]))
+    f_fetch = copy_kib / max(fetch[cal])
+    cal_w = max(write, key=lambda k: max(write[k]))
+    f_write = copy_kib / max(write[cal_w])
+    plain = "dam_break_1048576_2048"
+    batched = "batched 2D dam-break, 256 scenes"
+    out = {"_calibration": {"kernel": cal[:80], "copy_kib": copy_kib,
+                            "fetch_factor": f_fetch, "write_factor": f_write}}
+    for name in fetch:
+        for prefix, wl, key in KERNEL_KEYS:
+            if name.startswith(prefix) or name.startswith("void " + prefix):
+                fk = sum(fetch[name]) / len(fetch[name])
+                wk = sum(write.get(name, [0.0])) / max(1, len(write.get(name, [0.0])))
+                w = out.setdefault(plain if wl == "plain" else batched, {})
+                w[key] = {"bytes": (fk * f_fetch + wk * f_write) * 1024.0,
+                          "fetch_kib": fk, "write_kib": wk, "dispatches": len(fetch[name])}
+    dst = ROOT / "profiles" / "traffic.json"
+    dst.write_text(json.dumps(out, indent=1) + "\n")
+    print(f"wrote {dst}")
+    return 0
+
+
+def measure_copy(mib: int) -> dict:
+    """HBM ceiling on this box: a device-to-device copy of `mib` MiB (well
+    above the 256 MB last-level cache), read + write bytes over the median
+    of 5 copies.  The same copy is the calibration kernel of the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    ms = ts[len(ts) // 2]
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes": 2 * n * 4, "ms": ms, "gbs": 2 * n * 4 / (ms * 1e-3) / 1e9}
+
+
+def measure_batched(args) -> dict:
+    """The batched 2D form: 256 independent 2D SPH + 1D-wave scenes (N_s
+    log-uniform in [1k, 64k], columns log-uniform in [256, 4096] cells),
+    tuned by the library's autotuner, K steps replayed from a hipGraph from
+    the initial state with inputs resident; per-kernel device time from HIP
+    events over the same trajectory; the CPU oracle (OpenMP over scenes) on
+    a bounded sample of the same workload."""
+    import numpy as np
+
+    from coupledwateranimation_amd import batch2d
+
+    steps = args.batched_steps
+    spec = batch2d.batch_c3(steps=steps)
+    n, nc = spec.n_total, spec.wave_total
+    cell_sub = int((spec.wave_len * spec.wave_substeps).sum())
+    b = spec.make()
+    t_tune = time.perf_counter()
+    tuning = b.tune(spec.dt, 3)
+    t_tune = time.perf_counter() - t_tune
+    print(f"bench: batched {spec.nscenes} scenes, {n} particles, {nc} wave cells; "
+          f"tuned in {t_tune:.2f} s: {tuning}", flush=True)
+    b.step(spec.dt, 5, graph=True)  # warm-up (graph instantiated, caches warm)
+    times = []
+    for _ in range(max(1, min(args.repeats, 3))):
+        b.reinit(spec.xv, spec.enabled, spec.h0, spec.h1)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(spec.dt, steps, graph=True)
+        b.sync()
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    g = b.slots()
+    finite = bool(np.isfinite(g["xv"]).all() and np.isfinite(g["h0"]).all())
+    # per-kernel times over the same trajectory
+    b.reinit(spec.xv, spec.enabled, spec.h0, spec.h1)
+    acc = {}
+    for _ in range(steps):
+        for k, v in b.step_profiled(spec.dt).items():
+            acc[k] = acc.get(k, 0.0) + v
+    kernel_ms = {k: v / steps for k, v in acc.items()}
+    b.close()
+    alg = {k: BYTES_PER_PARTICLE_2D[k] * n for k in BYTES_PER_PARTICLE_2D}
+    alg["wave"] = BYTES_PER_WAVE_CELL_2D * nc + 4 * n
+    alg["scan"] = BYTES_PER_CELL * (spec.nscenes * g["nx"] * g["ny"])
+    dominant = max(kernel_ms, key=lambda k: kernel_ms[k])
+    achieved = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+    step_bytes = sum(alg.values())
+    ms_step = 1e3 * t / steps
+    print(f"bench: batched {steps} steps in {t:.3f} s ({ms_step:.4f} ms/step); kernels "
+          + ", ".join(f"{k} {v:.4f}" for k, v in kernel_ms.items()), flush=True)
+    cpu = None
+    if args.batched_cpu_steps > 0:
+        from oracle import oracle2d
+
+        threads = min(args.cpu_threads or len(os.sched_getaffinity(0)), 16)
+        o = oracle2d.OracleBatch(spec, threads=threads)
+        c0 = time.perf_counter()
+        o.step(spec.dt, args.batched_cpu_steps)
+        ct = time.perf_counter() - c0
+        o.close()
+        cpu = {"value": n * args.batched_cpu_steps / ct, "unit": "particle-updates/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.batched_cpu_steps} step(s) of the same 256-scene batch from "
+                         f"its initial state, fp32 oracle, OpenMP over scenes, {ct:.2f} s"}
+        print(f"bench: batched CPU oracle {threads} threads: {ct:.2f} s", flush=True)
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text())[spec.name][dominant]["bytes"]
+        except (ValueError, KeyError, TypeError):
+            traffic = None
+    return {
+        "workload": spec.name, "scenes": spec.nscenes, "particles": n, "wave_cells": nc,
+        "steps": steps, "dt": spec.dt, "ms_per_step": ms_step,
+        "repeats_ms_per_step": [1e3 * x / steps for x in times],
+        "particle_updates_per_s": n * steps / t,
+        "wave_cell_updates_per_s": cell_sub * steps / t,
+        "kernel_ms": kernel_ms, "tuning": tuning, "tune_s": t_tune,
+        "step_bytes": step_bytes, "step_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic, "alg_bytes": alg[dominant]},
+        "cpu_baseline": cpu, "finite": finite,
+    }
 
 
 def measure_adjoint(args) -> dict:
@@ -95,6 +289,8 @@ def measure_adjoint(args) -> dict:
 
 def main() -> int:
     args = parse()
+    if args.write_traffic:
+        return write_traffic(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the coupled step runs on a single device "
               "(multi-GPU sharding is a design note, not implemented); use --gpus 1",
@@ -118,18 +314,27 @@ def main() -> int:
     graph = not args.eager
 
     print(f"bench: {spec.name}: {n} particles, {W}x{H} wave, {ncell} grid cells", flush=True)
-    scene.step(spec.dt, args.warmup, graph=graph)
-    scene.sync()
     import torch  # noqa: F401  (device synchronisation on both sides)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    scene.step(spec.dt, args.steps, graph=graph)
-    scene.sync()
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
+    # The same trajectory `repeats` times: back to the initial state, W
+    # untimed warm-up steps, then K timed steps; the value is the median.
+    times = []
+    for r in range(max(1, args.repeats)):
+        if r:
+            scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+        scene.step(spec.dt, args.warmup, graph=graph)
+        scene.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.step(spec.dt, args.steps, graph=graph)
+        scene.sync()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
     ms_step = 1e3 * t / args.steps
-    print(f"bench: timed {args.steps} steps in {t:.3f} s ({ms_step:.4f} ms/step)", flush=True)
+    print(f"bench: timed {args.steps} steps x {len(times)} repeats, median {t:.3f} s "
+          f"({ms_step:.4f} ms/step), min {1e3 * min(times) / args.steps:.4f}, "
+          f"max {1e3 * max(times) / args.steps:.4f}", flush=True)
 
     st = scene.readback()
     finite = bool(np.isfinite(st["pos"]).all() and np.isfinite(st["height0"]).all())
@@ -161,11 +366,16 @@ def main() -> int:
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(spec.name, {}).get(dominant)
-        except (ValueError, AttributeError):
+            traffic = json.loads(tf.read_text())[spec.name][dominant]["bytes"]
+        except (ValueError, KeyError, TypeError):
             traffic = None
 
+    copy = measure_copy(args.copy_mib) if args.copy_mib > 0 else None
+    if copy:
+        print(f"bench: device copy {copy['bytes'] / 2**30:.1f} GiB moved in {copy['ms']:.3f} ms: "
+              f"{copy['gbs']:.0f} GB/s", flush=True)
     adjoint = measure_adjoint(args) if args.adjoint_steps > 0 else None
+    batched = measure_batched(args) if args.batched_steps > 0 else None
 
     cpu = None
     if args.cpu_steps > 0:
@@ -208,6 +418,8 @@ def main() -> int:
         "step_bytes": step_bytes,
         "step_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
         "kernel_ms": kernel_ms,
+        "repeats_ms_per_step": [1e3 * x / args.steps for x in times],
+        "copy_ceiling": copy,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
@@ -216,6 +428,7 @@ def main() -> int:
         "finite": finite,
         "state_after": state_stats,
         "adjoint": adjoint,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     return 0
