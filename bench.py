@@ -12,8 +12,9 @@ roofline of the dominant kernel (per-kernel average device time from HIP
 events recorded on the scene's stream; HBM bytes per launch from the
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/traffic.json), the
 measured device-copy ceiling, the forward+adjoint / forward ratio, the
-batched 2D form (256 scenes, tuned, graph-replayed) and the CPU oracle
-(OpenMP) timed on bounded samples of the same workloads.
+batched 2D form (256 scenes, tuned, graph-replayed), the batched 3D
+parameter fit (64 scenes, forward + adjoint in one hipGraph per iteration)
+and the CPU oracle (OpenMP) timed on bounded samples of the same workloads.
 
 `--write-traffic FETCH_CSV WRITE_CSV` turns two rocprofv3 --pmc counter
 collections of a bench run into profiles/traffic.json.
@@ -72,6 +73,9 @@ def parse():
                     help="steps of the batched 2D measurement (0 disables it)")
     ap.add_argument("--batched-cpu-steps", type=int, default=1,
                     help="oracle steps of the batched CPU baseline (0 disables it)")
+    ap.add_argument("--fit-iters", type=int, default=50,
+                    help="gradient-descent iterations of the batched fit (0 disables it)")
+    ap.add_argument("--fit-scenes", type=int, default=64)
     ap.add_argument("--repeats", type=int, default=5,
                     help="timed repeats of the K steps (value: the median)")
     ap.add_argument("--copy-mib", type=int, default=2048,
@@ -160,6 +164,53 @@ def measure_copy(mib: int) -> dict:
     del a, b
     torch.cuda.empty_cache()
     return {"bytes": 2 * n * 4, "ms": ms, "gbs": 2 * n * 4 / (ms * 1e-3) / 1e9}
+
+
+def measure_fit(args) -> dict:
+    """Batched forward+adjoint parameter fits: `--fit-scenes` 3D scenes
+    (N_s log-uniform in [8k, 128k], heightfield sides uniform in
+    [128, 1024], seed 4), 128 steps checkpointed every 16, all scenes'
+    forward, loss and adjoint in one hipGraph replayed per iteration; six
+    scalars per scene fitted by Adam toward seeded target heightfields."""
+    import numpy as np
+
+    from coupledwateranimation_amd import fit
+
+    t0 = time.perf_counter()
+    specs, targets, truth = fit.fit_workload(nscenes=args.fit_scenes)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run = fit.FitRun(specs, targets)
+    t_capture = time.perf_counter() - t0
+    res = run.descend(iters=args.fit_iters)
+    run.close()
+    steps = specs[0].steps
+    n = np.array([sp.n for sp in specs])
+    evals = np.array(res.eval_s[1:] if len(res.eval_s) > 1 else res.eval_s)
+    t_eval = float(np.median(evals))
+    ratio = res.losses[-1] / np.maximum(res.losses[0], 1e-300)
+    best = res.best_loss / np.maximum(res.losses[0], 1e-300)
+    err0 = np.abs(1.0 - truth).mean(axis=1)
+    err1 = np.abs(res.best_u - truth).mean(axis=1)
+    per_scene = res.tape_bytes + res.scene_bytes
+    print(f"bench: fit {len(specs)} scenes, {int(n.sum())} particles, {args.fit_iters} iterations: "
+          f"eval {1e3 * t_eval:.1f} ms, best/initial loss median {np.median(best):.3e}", flush=True)
+    return {
+        "workload": f"{len(specs)} 3D scenes, 8k-128k particles, 128^2-1024^2 heightfields, "
+                    f"{steps} steps, checkpoint every 16",
+        "scenes": len(specs), "particles": int(n.sum()), "steps": steps,
+        "iterations": args.fit_iters, "setup_s": t_setup, "capture_s": t_capture,
+        "eval_ms": 1e3 * t_eval,
+        "fwd_adjoint_particle_steps_per_s": float(n.sum()) * steps / t_eval,
+        "loss_ratio_last": {"median": float(np.median(ratio)), "max": float(ratio.max()),
+                            "min": float(ratio.min())},
+        "loss_ratio_best": {"median": float(np.median(best)), "max": float(best.max()),
+                            "min": float(best.min())},
+        "param_error": {"initial_mean": float(err0.mean()), "best_mean": float(err1.mean()),
+                        "note": "mean |theta/theta0 - true factor| over the six scalars"},
+        "hbm_bytes_per_scene": {"median": float(np.median(per_scene)),
+                                "max": float(per_scene.max()), "total": float(per_scene.sum())},
+    }
 
 
 def measure_batched(args) -> dict:
@@ -376,6 +427,7 @@ def main() -> int:
               f"{copy['gbs']:.0f} GB/s", flush=True)
     adjoint = measure_adjoint(args) if args.adjoint_steps > 0 else None
     batched = measure_batched(args) if args.batched_steps > 0 else None
+    fitres = measure_fit(args) if args.fit_iters > 0 else None
 
     cpu = None
     if args.cpu_steps > 0:
@@ -429,6 +481,7 @@ def main() -> int:
         "state_after": state_stats,
         "adjoint": adjoint,
         "batched": batched,
+        "fit": fitres,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -200,6 +200,12 @@ SIGNATURES = {
                                   C.POINTER(C.c_int32), C.c_uint64]),
     "cwa_gen_bumps": (C.c_int, [C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_uint64]),
+    "cwa_fit_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.POINTER(C.c_float)), C.c_int32, C.c_int32,
+                                 C.c_float, C.POINTER(C.c_void_p)]),
+    "cwa_fit_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(Grad)]),
+    "cwa_fit_tape_bytes": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
+    "cwa_fit_destroy": (C.c_int, [C.c_void_p]),
     # batched 2D (include/cwa2d.h)
     "cwa2d_abi_version": (C.c_int, []),
     "cwa2d_struct_size": (C.c_int, [C.c_int]),

@@ -148,7 +148,7 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
 // G0n = adjoint of h^n (before particle-sample contributions) and
 // G1n = adjoint of h^(n-1).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
+__global__ void __launch_bounds__(256) k_adj_wave(const DevParams* __restrict__ Pd, float dt,
                                                   const float* __restrict__ H0,
                                                   const float* __restrict__ H1,
                                                   const float* __restrict__ colflux,
@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
                                                   float* __restrict__ G0n,
                                                   float* __restrict__ G1n,
                                                   double* __restrict__ acc) {
+    const DevParams& P = *Pd;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = P.W, H = P.H;
@@ -203,9 +204,10 @@ __global__ void __launch_bounds__(256) k_adj_wave(DevParams P, float dt,
 // column visits the texels whose sample can touch it (texel -> column map
 // is affine), in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_colflux(DevParams P, float dt,
+__global__ void __launch_bounds__(256) k_adj_colflux(const DevParams* __restrict__ Pd, float dt,
                                                      const float* __restrict__ G0,
                                                      float* __restrict__ Sbar) {
+    const DevParams& P = *Pd;
     // one wave64 per column; lanes stride over the column's texel window
     const int col = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -263,7 +265,7 @@ __global__ void k_reduce_partials(const double* __restrict__ acc, int nb, int np
 // Writes F-bar and V-bar (the viscosity sum's adjoint, w = crest factor),
 // the direct rho-bar, and the local parts of x-bar / v-bar.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_local(DevParams P, float dt, int n,
+__global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__ Pd, float dt, int n,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel,
                                                    const float4* __restrict__ fpre,
@@ -276,6 +278,7 @@ __global__ void __launch_bounds__(256) k_adj_local(DevParams P, float dt, int n,
                                                    float* __restrict__ rhob_d,
                                                    float4* __restrict__ xb_loc,
                                                    float4* __restrict__ vb_loc) {
+    const DevParams& P = *Pd;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 x = pos[i], v = vel[i], F = fpre[i];
@@ -382,7 +385,7 @@ __global__ void __launch_bounds__(256) k_adj_local(DevParams P, float dt, int n,
 // adjoint of particle k (EOS, rest-density clamp, wave term, flux).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_pairs(
-    DevParams P, float dt, int n, const float4* __restrict__ pos,
+    const DevParams* __restrict__ Pd, float dt, int n, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const int* __restrict__ offset,
     const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
     const float* __restrict__ rho_w, const float2* __restrict__ pw,
@@ -390,6 +393,7 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
     const float* __restrict__ Sbar, const float4* __restrict__ adjrec,
     const float* __restrict__ rhob_d, float* __restrict__ G0n, float4* __restrict__ xb_loc,
     float4* __restrict__ vb_loc, float* __restrict__ rtb, double* __restrict__ acc) {
+    const DevParams& P = *Pd;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float part[3] = {0.f, 0.f, 0.f};  // visc, gas_const, rest_rho
     if (k < n) {
@@ -516,10 +520,11 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
 // (the forward reorder sent slot list[k] to sorted position k).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_rho_pairs(
-    DevParams P, int n, const float4* __restrict__ pos, const int* __restrict__ offset,
+    const DevParams* __restrict__ Pd, int n, const float4* __restrict__ pos, const int* __restrict__ offset,
     const uint2* __restrict__ nbmask, const float* __restrict__ rtb,
     const float4* __restrict__ xb_loc, const float4* __restrict__ vb_loc,
     const int* __restrict__ list, float4* __restrict__ xb_out, float4* __restrict__ vb_out) {
+    const DevParams& P = *Pd;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 xk = pos[k];
@@ -549,18 +554,58 @@ static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 int adj_wave_blocks(const DevParams& P) { return ((P.W + 63) / 64) * ((P.H + 3) / 4); }
 int adj_pair_blocks(int n) { return nblk(n, 256); }
 
+// Loss seed of a heightfield fit: L = mean((h - t)^2), dL/dh = 2 (h - t) / N.
+// Per-block partial sums (double, fixed order: grid-stride per thread, then
+// an LDS tree), summed by one thread in block order.
+__global__ void __launch_bounds__(256) k_mse_partials(const float* __restrict__ h,
+                                                      const float* __restrict__ t, int N,
+                                                      float* __restrict__ G,
+                                                      double* __restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    const double scale = 2.0 / (double)N;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < N; k += gridDim.x * 256) {
+        const double d = (double)h[k] - (double)t[k];  // exact for two floats
+        G[k] = (float)(scale * d);
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_mse_total(const double* __restrict__ part, int nb, int N,
+                            double* __restrict__ loss) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    *loss = s / (double)N;
+}
+
+hipError_t launch_mse_seed(const float* h, const float* target, int N, float* G, double* part,
+                           int nblocks, double* loss, hipStream_t st) {
+    if (N <= 0 || nblocks <= 0) return hipErrorInvalidValue;
+    k_mse_partials<<<nblocks, 256, 0, st>>>(h, target, N, G, part);
+    k_mse_total<<<1, 64, 0, st>>>(part, nblocks, N, loss);
+    return hipGetLastError();
+}
+
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,
                            const float* colflux, const float* G0, const float* G1, float* G0n,
                            float* G1n, double* acc, hipStream_t st) {
     dim3 blk(64, 4), grd((P.W + 63) / 64, (P.H + 3) / 4);
-    k_adj_wave<<<grd, blk, 0, st>>>(P, dt, H0, H1, colflux, G0, G1, G0n, G1n, acc);
+    k_adj_wave<<<grd, blk, 0, st>>>(P.dev, dt, H0, H1, colflux, G0, G1, G0n, G1n, acc);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_colflux(const DevParams& P, float dt, const float* G0, float* Sbar,
                               hipStream_t st) {
     const long long ncol = (long long)P.dims[0] * P.dims[2];
-    k_adj_colflux<<<nblk(ncol * 64, 256), 256, 0, st>>>(P, dt, G0, Sbar);
+    k_adj_colflux<<<nblk(ncol * 64, 256), 256, 0, st>>>(P.dev, dt, G0, Sbar);
     return hipGetLastError();
 }
 
@@ -578,7 +623,7 @@ hipError_t launch_adj_local(const DevParams& P, float dt, int n, const float4* p
                             float* G0n, float4* adjrec, float* rhob_d, float4* xb_loc,
                             float4* vb_loc, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_local<<<nblk(n, 256), 256, 0, st>>>(P, dt, n, pos, vel, fpre, rho_w, H0, xb_in, vb_in,
+    k_adj_local<<<nblk(n, 256), 256, 0, st>>>(P.dev, dt, n, pos, vel, fpre, rho_w, H0, xb_in, vb_in,
                                               G0n, adjrec, rhob_d, xb_loc, vb_loc);
     return hipGetLastError();
 }
@@ -591,7 +636,7 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P, dt, n, pos, vel, offset, nbmask, nrec, rho_w,
+    k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, dt, n, pos, vel, offset, nbmask, nrec, rho_w,
                                               pw, rho_raw, H0, Sbar, adjrec, rhob_d, G0n,
                                               xb_loc, vb_loc, rtb, acc);
     return hipGetLastError();
@@ -602,7 +647,7 @@ hipError_t launch_adj_rho_pairs(const DevParams& P, int n, const float4* pos, co
                                 const float4* vb_loc, const int* list, float4* xb_out,
                                 float4* vb_out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_rho_pairs<<<nblk(n, 256), 256, 0, st>>>(P, n, pos, offset, nbmask, rtb, xb_loc, vb_loc,
+    k_adj_rho_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, n, pos, offset, nbmask, rtb, xb_loc, vb_loc,
                                                   list, xb_out, vb_out);
     return hipGetLastError();
 }

@@ -24,6 +24,9 @@ enum GradSlot {
 // arrays of blocks x slots; launch_reduce_partials sums them in block order.
 int adj_wave_blocks(const DevParams& P);  // blocks of launch_adj_wave (5 slots each)
 int adj_pair_blocks(int n);               // blocks of launch_adj_pairs (3 slots each)
+// MSE of a heightfield against a target: seed G = 2 (h - t)/N, loss = mean.
+hipError_t launch_mse_seed(const float* h, const float* target, int N, float* G, double* part,
+                           int nblocks, double* loss, hipStream_t st);
 hipError_t launch_reduce_partials(const double* acc, int nb, int np, const int* slots,
                                   double* pgrad, hipStream_t st);
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,

@@ -47,6 +47,10 @@ struct DevParams {
     float cell;
     int dims[3];
     int num_cells;
+    // Device copy of this block.  Kernels read their parameters through it,
+    // so a captured graph replays with whatever the host last uploaded
+    // (parameter fits change them between replays).
+    const struct DevParams* dev;
 };
 
 // Squared distance as the fused chain fma(dz, dz, fma(dy, dy, dx*dx)); the

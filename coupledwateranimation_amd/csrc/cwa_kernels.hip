@@ -332,7 +332,7 @@ __device__ __forceinline__ void density_candidate(const float4 pi, const float4 
 // column a 64-bit mask over its first 64 candidates (nbmask[c*n + i];
 // candidates past 64 are re-tested by the force pass).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
                                                  const int* __restrict__ offset, int n,
                                                  const float* __restrict__ H0,
@@ -342,6 +342,7 @@ __global__ void __launch_bounds__(256) k_density(DevParams P, const float4* __re
                                                  uint2* __restrict__ nbmask,
                                                  float4* __restrict__ nrec,
                                                  float* __restrict__ rho_raw) {
+    const DevParams& P = *Pd;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
@@ -447,13 +448,14 @@ __device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4
 // copy B and writes the state A in the same slot order.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_force_integrate(
-    DevParams P, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const DevParams* __restrict__ Pd, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
     int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
     const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
     const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
     float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre) {
+    const DevParams& P = *Pd;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
@@ -602,11 +604,12 @@ __global__ void __launch_bounds__(256) k_column_flux(const int* __restrict__ off
 // neighbours, optional particle flux source, wake line.  Arithmetic is not
 // contracted so the result matches the oracle bit for bit.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_wave(DevParams P, const float* __restrict__ h0,
+__global__ void __launch_bounds__(256) k_wave(const DevParams* __restrict__ Pd, const float* __restrict__ h0,
                                               const float* __restrict__ h1,
                                               float* __restrict__ out,
                                               const float* __restrict__ colflux, float dt) {
 #pragma clang fp contract(off)
+    const DevParams& P = *Pd;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = P.W, H = P.H;
@@ -648,10 +651,11 @@ __device__ __forceinline__ float wave_texel(const DevParams& P, float C0, float 
     return o;
 }
 
-__global__ void __launch_bounds__(256) k_wave4(DevParams P, const float* __restrict__ h0,
+__global__ void __launch_bounds__(256) k_wave4(const DevParams* __restrict__ Pd, const float* __restrict__ h0,
                                                const float* __restrict__ h1,
                                                float* __restrict__ out,
                                                const float* __restrict__ colflux, float dt) {
+    const DevParams& P = *Pd;
     const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = P.W, H = P.H;
@@ -739,7 +743,7 @@ hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
                           float4* nrec, float* rho_raw, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_density<<<blocks_for(n, 256), 256, 0, st>>>(P, B.pos, B.vel, offset, n, H0, rho_w, pw,
+    k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, B.pos, B.vel, offset, n, H0, rho_w, pw,
                                                   flux, nbmask, nrec, rho_raw);
     return hipGetLastError();
 }
@@ -750,7 +754,7 @@ hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleAr
                                   ParticleArrays& A, float* rho, float* prs, float4* fpre,
                                   hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P, dt, B.pos, B.vel, B.force, B.id,
+    k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, dt, B.pos, B.vel, B.force, B.id,
                                                           offset, n, rho_w, pw, nbmask, nrec, H0,
                                                           A.pos, A.vel, A.force, A.id, rho, prs,
                                                           fpre);
@@ -772,10 +776,10 @@ hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, flo
     dim3 blk(64, 4);
     if (P.W % 4 == 0) {
         dim3 grd((P.W / 4 + 63) / 64, (P.H + 3) / 4);
-        k_wave4<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+        k_wave4<<<grd, blk, 0, st>>>(P.dev, h0, h1, out, colflux, dt);
     } else {
         dim3 grd((P.W + 63) / 64, (P.H + 3) / 4);
-        k_wave<<<grd, blk, 0, st>>>(P, h0, h1, out, colflux, dt);
+        k_wave<<<grd, blk, 0, st>>>(P.dev, h0, h1, out, colflux, dt);
     }
     return hipGetLastError();
 }

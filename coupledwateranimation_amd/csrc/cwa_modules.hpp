@@ -137,6 +137,14 @@ class UniformGrid3D {
     // each build's scan clears the other one).
     void ClearCounters(hipStream_t st);
     int parity() const { return cur_; }
+    void get_parity(int& cur, int& last) const {
+        cur = cur_;
+        last = last_;
+    }
+    void set_parity(int cur, int last) {
+        cur_ = cur;
+        last_ = last;
+    }
 
    private:
     GridDesc g_{};
@@ -164,6 +172,16 @@ class HeightField2DTripleBuffered {
     void PingPong();
     // rotation state (which buffer is read 0 / read 1 / write), 0..2
     int phase() const { return read_[0]; }
+    void get_rotation(int r[3]) const {
+        r[0] = read_[0];
+        r[1] = read_[1];
+        r[2] = write_;
+    }
+    void set_rotation(const int r[3]) {
+        read_[0] = r[0];
+        read_[1] = r[1];
+        write_ = r[2];
+    }
     float* GetReadImage(int i) const { return img_[read_[i]].get(); }
     float* GetWriteImage() const { return img_[write_].get(); }
     int width() const { return w_; }
@@ -261,6 +279,23 @@ class CoupledScene : public Module {
     // Host-side phase of the buffer rotations (wave triple buffer x grid
     // counter double buffer), period 6.
     int phase() const { return wave.phase() * 2 + grid.parity(); }
+    // The host-side rotation state itself: a captured sequence replays the
+    // buffers of the rotation it was captured at, so its owner puts the
+    // scene back into that rotation around replays.
+    struct Rotation {
+        int wave[3];
+        int grid_cur, grid_last;
+    };
+    Rotation rotation() const {
+        Rotation r{};
+        wave.get_rotation(r.wave);
+        grid.get_parity(r.grid_cur, r.grid_last);
+        return r;
+    }
+    void set_rotation(const Rotation& r) {
+        wave.set_rotation(r.wave);
+        grid.set_parity(r.grid_cur, r.grid_last);
+    }
 
     SphParticles3D particles;
     UniformGrid3D grid;
@@ -272,6 +307,7 @@ class CoupledScene : public Module {
     void invalidate_graph();
     cwa_params params_{};
     DevParams dp_{};
+    DeviceBuffer<DevParams> dpd_;  // device mirror read by the kernels
     hipStream_t st_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     float graph_dt_ = 0.0f;

@@ -363,7 +363,14 @@ void CoupledScene::invalidate_graph() {
     }
 }
 
-void CoupledScene::build_devparams() { dp_ = make_devparams(params_, wave.width(), wave.height()); }
+void CoupledScene::build_devparams() {
+    dp_ = make_devparams(params_, wave.width(), wave.height());
+    dpd_.resize(1);
+    dp_.dev = dpd_.get();
+    // synchronous: no kernel of this scene is in flight here (callers have
+    // synchronised the stream), and the next launch must see the new block
+    hip_check(hipMemcpy(dpd_.get(), &dp_, sizeof(DevParams), hipMemcpyHostToDevice), "params");
+}
 
 void CoupledScene::Reinit(const cwa_init& init) {
     validate_init(init);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/cwa.h"
@@ -26,6 +27,17 @@ class Tape {
     void Forward(float dt);
     void Backward(const float* dH0, const float* dH1, const float* dpos, const float* dvel,
                   double* grad, float* dpos0, float* dvel0);
+
+    // Device-only passes (no host transfer or synchronisation, so a graph
+    // can capture them): the forward from the saved initial state, the MSE
+    // seed of the newest heightfield against a device target, and the
+    // adjoint sweep leaving the parameter gradients in grad_dev().
+    void SaveInitial();
+    void ForwardFromInitial(float dt);
+    void SeedMse(const float* target, double* partials, int nblocks, double* loss);
+    void BackwardSeeded();
+    const double* grad_dev() const { return pgrad_.get(); }
+    int texels() const { return W_ * H_; }
 
    private:
     struct Ckpt {
@@ -139,6 +151,61 @@ void Tape::restore(const Ckpt& c) {
     copy(S_.wave.GetReadImage(0), c.h0.get(), (size_t)W_ * H_);
     copy(S_.wave.GetReadImage(1), c.h1.get(), (size_t)W_ * H_);
     S_.grid.ClearCounters(st_);
+}
+
+void Tape::SaveInitial() {
+    save(ck_[0]);
+    hip_check(hipStreamSynchronize(st_), "tape initial state");
+}
+
+void Tape::ForwardFromInitial(float dt) {
+    dt_ = dt;
+    restore(ck_[0]);
+    int done = 0;
+    for (int seg = 0; seg < nseg_; ++seg) {
+        if (seg > 0) save(ck_[seg]);
+        const int len = std::min(every_, nsteps_ - done);
+        for (int k = 0; k < len; ++k) S_.Step(dt);
+        done += len;
+    }
+    save(ck_[nseg_]);
+    forward_done_ = true;
+}
+
+void Tape::SeedMse(const float* target, double* partials, int nblocks, double* loss) {
+    const size_t n = (size_t)std::max(n_, 1);
+    hip_check(launch_mse_seed(S_.wave.GetReadImage(0), target, W_ * H_, G0_.get(), partials,
+                              nblocks, loss, st_),
+              "mse seed");
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(G1_.get()), W_ * H_, 0, st_), "seed h1");
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(xb_.get()), (int)(4 * n), 0, st_), "seed x");
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(vb_.get()), (int)(4 * n), 0, st_), "seed v");
+}
+
+void Tape::BackwardSeeded() {
+    if (!forward_done_) throw Error(CWA_ESTATE, "tape backward before forward");
+    auto zero = [&](DeviceBuffer<double>& b) {
+        hip_check(launch_fill_i32(reinterpret_cast<int*>(b.get()), (int)(2 * b.size()), 0, st_),
+                  "zero");
+    };
+    zero(pgrad_);
+    zero(acc_wave_);
+    zero(acc_pair_);
+    for (int seg = nseg_ - 1; seg >= 0; --seg) {
+        restore(ck_[seg]);
+        const int len = std::min(every_, nsteps_ - seg * every_);
+        for (int k = 0; k < len; ++k) record_step(tp_[k]);
+        for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k]);
+    }
+    const int wslots[5] = {GRAD_LAMBDA_X, GRAD_LAMBDA_Z, GRAD_ATTEN, GRAD_BETA, GRAD_KAPPA};
+    const int pslots[3] = {GRAD_VISC, GRAD_GAS_CONST, GRAD_REST_RHO};
+    hip_check(launch_reduce_partials(acc_wave_.get(), (int)(acc_wave_.size() / 5), 5, wslots,
+                                     pgrad_.get(), st_),
+              "reduce");
+    hip_check(launch_reduce_partials(acc_pair_.get(), (int)(acc_pair_.size() / 3), 3, pslots,
+                                     pgrad_.get(), st_),
+              "reduce");
+    restore(ck_[nseg_]);
 }
 
 void Tape::Forward(float dt) {
@@ -286,12 +353,147 @@ void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const
     download(vb_, dvel0);
 }
 
+// ---------------------------------------------------------------------------
+// FitBatch: gradients of a heightfield fit for many scenes at once.  For
+// every scene: the checkpointed forward from its saved initial state, the
+// MSE of its newest heightfield against a target, and the adjoint sweep;
+// all scenes are captured once into one hipGraph (each scene on its own
+// stream, forked from and joined into one capture stream) and the graph is
+// replayed per evaluation.  The kernels read their parameters from device
+// memory, so cwa_scene_set_params between evaluations takes effect without
+// a new capture (the grid and code paths must stay as captured).
+// ---------------------------------------------------------------------------
+class FitBatch {
+   public:
+    static constexpr int kMseBlocks = 256;
+
+    FitBatch(std::vector<CoupledScene*> scenes, const float* const* targets, int nsteps,
+             int every, float dt)
+        : sc_(std::move(scenes)) {
+        const int ns = (int)sc_.size();
+        if (ns < 1) throw Error(CWA_EINVAL, "nscenes must be >= 1");
+        if (!(dt > 0.0f)) throw Error(CWA_EINVAL, "dt must be positive");
+        for (int s = 0; s < ns; ++s) {
+            if (!sc_[s]) throw Error(CWA_EINVAL, "null scene");
+            if (sc_[s]->params().check_finite)
+                throw Error(CWA_EINVAL, "fit scenes must have check_finite == 0");
+            if (!targets || !targets[s]) throw Error(CWA_EINVAL, "null target");
+            for (int q = 0; q < s; ++q)
+                if (sc_[q] == sc_[s]) throw Error(CWA_EINVAL, "a scene appears twice");
+        }
+        hip_check(hipStreamCreateWithFlags(&ms_, hipStreamNonBlocking), "stream");
+        hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "event");
+        ev_.assign(ns, nullptr);
+        for (auto& e : ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        tapes_.resize(ns);
+        target_.resize(ns);
+        bytes_.assign(ns, 0);
+        for (int s = 0; s < ns; ++s) {
+            size_t f0 = 0, f1 = 0, tot = 0;
+            hip_check(hipMemGetInfo(&f0, &tot), "mem info");
+            tapes_[s].reset(new Tape(*sc_[s], nsteps, every));
+            const int tex = tapes_[s]->texels();
+            target_[s].resize(tex);
+            hip_check(hipMemcpy(target_[s].get(), targets[s], sizeof(float) * tex,
+                                hipMemcpyHostToDevice),
+                      "target");
+            hip_check(hipMemGetInfo(&f1, &tot), "mem info");
+            bytes_[s] = f0 > f1 ? (long long)(f0 - f1) : 0;
+            tapes_[s]->SaveInitial();
+        }
+        part_.resize((size_t)ns * kMseBlocks);
+        loss_.resize(ns);
+        Capture(dt);
+    }
+
+    ~FitBatch() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (ms_) {
+            (void)hipStreamSynchronize(ms_);
+            (void)hipStreamDestroy(ms_);
+        }
+        for (auto e : ev_)
+            if (e) (void)hipEventDestroy(e);
+        if (fork_) (void)hipEventDestroy(fork_);
+    }
+
+    void Run(double* losses, double* grads) {
+        hip_check(hipGraphLaunch(exec_, ms_), "fit graph");
+        const int ns = (int)sc_.size();
+        std::vector<double> g((size_t)ns * GRAD_NUM), l(ns);
+        for (int s = 0; s < ns; ++s)
+            hip_check(hipMemcpyAsync(g.data() + (size_t)s * GRAD_NUM, tapes_[s]->grad_dev(),
+                                     sizeof(double) * GRAD_NUM, hipMemcpyDeviceToHost, ms_),
+                      "grads");
+        hip_check(hipMemcpyAsync(l.data(), loss_.get(), sizeof(double) * ns,
+                                 hipMemcpyDeviceToHost, ms_),
+                  "losses");
+        hip_check(hipStreamSynchronize(ms_), "fit");
+        // the device state now matches the rotation the capture ended in
+        for (int s = 0; s < ns; ++s) sc_[s]->set_rotation(rot_end_[s]);
+        if (losses) std::memcpy(losses, l.data(), sizeof(double) * ns);
+        if (grads) std::memcpy(grads, g.data(), sizeof(double) * g.size());
+    }
+
+    int nscenes() const { return (int)sc_.size(); }
+    long long TapeBytes(int s) const {
+        if (s < 0 || s >= (int)sc_.size()) throw Error(CWA_EINVAL, "scene index out of range");
+        return bytes_[s];
+    }
+
+   private:
+    void Capture(float dt) {
+        const int ns = (int)sc_.size();
+        for (int s = 0; s < ns; ++s) hip_check(hipStreamSynchronize(sc_[s]->stream()), "sync");
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(ms_, hipStreamCaptureModeThreadLocal), "capture");
+        try {
+            hip_check(hipEventRecord(fork_, ms_), "fork");
+            for (int s = 0; s < ns; ++s) {
+                hipStream_t st = sc_[s]->stream();
+                hip_check(hipStreamWaitEvent(st, fork_, 0), "fork wait");
+                tapes_[s]->ForwardFromInitial(dt);
+                tapes_[s]->SeedMse(target_[s].get(), part_.get() + (size_t)s * kMseBlocks,
+                                   kMseBlocks, loss_.get() + s);
+                tapes_[s]->BackwardSeeded();
+                hip_check(hipEventRecord(ev_[s], st), "join");
+                hip_check(hipStreamWaitEvent(ms_, ev_[s], 0), "join wait");
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(ms_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(ms_, &g), "end capture");
+        const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "fit graph instantiate");
+        rot_end_.resize(ns);
+        for (int s = 0; s < ns; ++s) rot_end_[s] = sc_[s]->rotation();
+    }
+
+    std::vector<CoupledScene*> sc_;
+    std::vector<std::unique_ptr<Tape>> tapes_;
+    std::vector<DeviceBuffer<float>> target_;
+    std::vector<long long> bytes_;
+    std::vector<CoupledScene::Rotation> rot_end_;
+    DeviceBuffer<double> part_, loss_;
+    hipStream_t ms_ = nullptr;
+    hipEvent_t fork_ = nullptr;
+    std::vector<hipEvent_t> ev_;
+    hipGraphExec_t exec_ = nullptr;
+};
+
 }  // namespace cwa
 
 using namespace cwa;
 
 struct cwa_tape {
     Tape* impl;
+};
+
+struct cwa_fit {
+    FitBatch* impl;
 };
 
 // defined in cwa_scene.hip
@@ -304,6 +506,17 @@ namespace {
 template <class F>
 int guarded_lambda(F&& f) {
     return cwa::guarded_call([](void* c) { (*static_cast<F*>(c))(); }, &f);
+}
+
+void copy_grad(const double* g, cwa_grad* out) {
+    out->visc = g[GRAD_VISC];
+    out->gas_const = g[GRAD_GAS_CONST];
+    out->rest_rho = g[GRAD_REST_RHO];
+    out->lambda_x = g[GRAD_LAMBDA_X];
+    out->lambda_z = g[GRAD_LAMBDA_Z];
+    out->atten = g[GRAD_ATTEN];
+    out->couple_strength = g[GRAD_KAPPA];
+    out->beta = g[GRAD_BETA];
 }
 }  // namespace
 
@@ -338,14 +551,51 @@ int cwa_tape_backward(cwa_tape* t, const float* dheight0, const float* dheight1,
         if (!t || !grad) throw Error(CWA_EINVAL, "null argument");
         double g[GRAD_NUM];
         t->impl->Backward(dheight0, dheight1, dpos, dvel, g, dpos0, dvel0);
-        grad->visc = g[GRAD_VISC];
-        grad->gas_const = g[GRAD_GAS_CONST];
-        grad->rest_rho = g[GRAD_REST_RHO];
-        grad->lambda_x = g[GRAD_LAMBDA_X];
-        grad->lambda_z = g[GRAD_LAMBDA_Z];
-        grad->atten = g[GRAD_ATTEN];
-        grad->couple_strength = g[GRAD_KAPPA];
-        grad->beta = g[GRAD_BETA];
+        copy_grad(g, grad);
+    });
+}
+
+int cwa_fit_create(int32_t nscenes, cwa_scene* const* scenes, const float* const* targets,
+                   int32_t nsteps, int32_t ckpt_every, float dt, cwa_fit** out) {
+    return guarded_lambda([&] {
+        if (!scenes || !targets || !out || nscenes < 1) throw Error(CWA_EINVAL, "bad argument");
+        *out = nullptr;
+        std::vector<CoupledScene*> sc(nscenes);
+        for (int s = 0; s < nscenes; ++s) sc[s] = scene_impl(scenes[s]);
+        auto* f = new cwa_fit{nullptr};
+        try {
+            f->impl = new FitBatch(std::move(sc), targets, nsteps, ckpt_every, dt);
+        } catch (...) {
+            delete f;
+            throw;
+        }
+        *out = f;
+    });
+}
+
+int cwa_fit_run(cwa_fit* f, double* losses, cwa_grad* grads) {
+    return guarded_lambda([&] {
+        if (!f || !f->impl) throw Error(CWA_EINVAL, "null fit");
+        const int ns = f->impl->nscenes();
+        std::vector<double> g((size_t)ns * GRAD_NUM);
+        f->impl->Run(losses, g.data());
+        if (grads)
+            for (int s = 0; s < ns; ++s) copy_grad(g.data() + (size_t)s * GRAD_NUM, grads + s);
+    });
+}
+
+int cwa_fit_tape_bytes(cwa_fit* f, int32_t scene, int64_t* bytes) {
+    return guarded_lambda([&] {
+        if (!f || !f->impl || !bytes) throw Error(CWA_EINVAL, "null argument");
+        *bytes = f->impl->TapeBytes(scene);
+    });
+}
+
+int cwa_fit_destroy(cwa_fit* f) {
+    return guarded_lambda([&] {
+        if (!f) return;
+        delete f->impl;
+        delete f;
     });
 }
 

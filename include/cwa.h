@@ -285,6 +285,26 @@ int cwa_tape_backward(cwa_tape* t, const float* dheight0, const float* dheight1,
                       float* dvel0);
 int cwa_tape_destroy(cwa_tape* t);
 
+/* ---- batched heightfield fits (no reference counterpart) ----
+ * For each of `nscenes` scenes: the checkpointed forward of `nsteps` steps
+ * from the scene's state at creation, the loss mean((h0 - target)^2) of its
+ * newest heightfield against `targets[s]` (H x W, row-major), and the
+ * adjoint sweep.  All scenes are captured into one hipGraph (each scene on
+ * its own stream) at creation; cwa_fit_run replays it and returns each
+ * scene's loss and parameter gradients.  cwa_scene_set_params between runs
+ * takes effect without a new capture as long as the grid, the coupling
+ * band and the wave mode stay as they were.  Scenes need torque_coeff == 0
+ * and check_finite == 0, must not be stepped or destroyed while the fit
+ * exists, and each scene may appear once. */
+typedef struct cwa_fit cwa_fit;
+int cwa_fit_create(int32_t nscenes, cwa_scene* const* scenes, const float* const* targets,
+                   int32_t nsteps, int32_t ckpt_every, float dt, cwa_fit** out);
+/* losses: nscenes doubles; grads: nscenes cwa_grad (either may be null). */
+int cwa_fit_run(cwa_fit* f, double* losses, cwa_grad* grads);
+/* Device bytes the fit allocated for one scene (tape, checkpoints, target). */
+int cwa_fit_tape_bytes(cwa_fit* f, int32_t scene, int64_t* bytes);
+int cwa_fit_destroy(cwa_fit* f);
+
 /* ---- scene generators (host only, no device use) ----
  * Random numbers: SplitMix64 stream from `seed`; u = (x >> 40) * 2^-24.
  *
