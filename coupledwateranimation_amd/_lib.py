@@ -126,9 +126,14 @@ class Params2D(C.Structure):
         return p
 
 
+CWA2D_WAVE_LEAPFROG = 0
+CWA2D_WAVE_SHALLOW = 1
+
+
 class Scene2D(C.Structure):
     _fields_ = [("n", C.c_int32), ("wave_len", C.c_int32), ("wave_substeps", C.c_int32),
-                ("mass", C.c_float), ("lambda_", C.c_float)]
+                ("wave_model", C.c_int32), ("mass", C.c_float), ("lambda_", C.c_float),
+                ("wave_dx", C.c_float)]
 
 
 class Tuning2D(C.Structure):
@@ -213,6 +218,10 @@ SIGNATURES = {
     "cwa2d_params_reference": (C.c_int, [C.POINTER(Params2D)]),
     "cwa2d_wave_init_reference": (C.c_int, [C.c_int32, C.POINTER(C.c_float),
                                             C.POINTER(C.c_float)]),
+    "cwa2d_wave_init_shallow": (C.c_int, [C.c_int32, C.c_float, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float)]),
+    "cwa2d_batch_splash": (C.c_int, [C.c_void_p, C.c_int32, C.c_float]),
+    "cwa2d_batch_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cwa2d_batch_create": (C.c_int, [C.POINTER(Params2D), C.c_int32, C.POINTER(Scene2D),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -271,7 +280,7 @@ def lib() -> C.CDLL:
         for k, st in enumerate(STRUCTS):
             if L.cwa_struct_size(k) != C.sizeof(st):
                 raise RuntimeError(f"struct {st.__name__} size mismatch with libcwa")
-        if L.cwa2d_abi_version() != 1:
+        if L.cwa2d_abi_version() != 2:
             raise RuntimeError("libcwa 2D ABI version mismatch")
         for k, st in enumerate(STRUCTS_2D):
             if L.cwa2d_struct_size(k) != C.sizeof(st):

@@ -45,6 +45,14 @@ def wave_init_reference(n: int) -> tuple[np.ndarray, np.ndarray]:
     return h0, h1
 
 
+def wave_init_shallow(n: int, view_height: float = 9.6) -> tuple[np.ndarray, np.ndarray]:
+    """Shallow1D InitWave (Shallow1D_cs.glsl:117-131): (h, uh)."""
+    h = np.zeros(n, np.float32)
+    uh = np.zeros(n, np.float32)
+    L.check(L.lib().cwa2d_wave_init_shallow(int(n), float(view_height), L.fptr(h), L.fptr(uh)))
+    return h, uh
+
+
 def psi_for(h: float, rest_rho: float) -> float:
     """PSI = REST_DENS / (1.5 k2), k2 = 40/(7 pi H^2), in float
     (SphWaveKoschier2D_grid_cs.glsl:135-136)."""
@@ -71,6 +79,8 @@ class BatchSpec:
     enabled: np.ndarray | None = None
     name: str = ""
     extra: dict = field(default_factory=dict)
+    wave_model: np.ndarray | None = None   # per scene; None: all leapfrog
+    wave_dx: np.ndarray | None = None      # per scene; shallow-water cell size
 
     @property
     def nscenes(self) -> int:
@@ -92,6 +102,8 @@ class BatchSpec:
             arr[s].wave_substeps = int(self.wave_substeps[s])
             arr[s].mass = float(self.mass[s])
             arr[s].lambda_ = float(self.lam[s])
+            arr[s].wave_model = 0 if self.wave_model is None else int(self.wave_model[s])
+            arr[s].wave_dx = 1.0 if self.wave_dx is None else float(self.wave_dx[s])
         return arr
 
     def pbeg(self) -> np.ndarray:
@@ -193,6 +205,15 @@ class Batch2D:
         t = L.Tuning2D()
         L.check(self._lib.cwa2d_batch_tune(self._h, float(dt), int(steps), C.byref(t)))
         return t.as_dict()
+
+    def splash(self, scene: int = -1, view_height: float = 9.6) -> None:
+        """The reference's Splash button on the shallow-water scenes."""
+        L.check(self._lib.cwa2d_batch_splash(self._h, int(scene), float(view_height)))
+
+    def render(self, points_xy_ptr: int | None, wave_ptr: int | None) -> None:
+        """Copy (x, y) per slot and the newest wave level into device memory
+        owned by the caller (e.g. torch tensors' data_ptr())."""
+        L.check(self._lib.cwa2d_batch_render(self._h, points_xy_ptr or None, wave_ptr or None))
 
     def views(self) -> L.Views2D:
         v = L.Views2D()
@@ -331,12 +352,15 @@ def small_batch(nscenes: int = 6, seed: int = 7, lo: int = 64, hi: int = 3000, s
                            name=f"small batch {nscenes}")
 
 
-def reference_batch(nscenes: int = 1, steps: int = 100) -> BatchSpec:
+def reference_batch(nscenes: int = 1, steps: int = 100, shallow: bool = False) -> BatchSpec:
     """The SphWave2D application's scene (SphWave2D/Main.cpp:42,63-75 and
     InitParticle / init_grid, SphWaveKoschier2D_grid_cs.glsl:366-391): 4096
-    particles on a 128 x 32 grid over (VIEW_WIDTH, 18 H), 1024 wave cells,
-    10 substeps, lambda = 0.01, DT = 0.002, 2 SPH substeps; `nscenes`
-    copies."""
+    particles on a 128 x 32 grid over (VIEW_WIDTH, 18 H), DT = 0.002, 2 SPH
+    substeps; `nscenes` copies.  Wave: Wave1D with 1024 cells, 10 substeps,
+    lambda = 0.01 (InitWaveEquation, Main.cpp:63-75), or with `shallow` the
+    wave the application actually runs (InitShallowWaterEquation,
+    Main.cpp:78-95,283): Shallow1D with 128 cells, one substep, lambda =
+    0.001, dx = 0.1, started from its InitWave."""
     p = reference_params()
     prad, view, H = 0.025, 9.6, p.h
     n, cols = 4096, 128
@@ -348,12 +372,19 @@ def reference_batch(nscenes: int = 1, steps: int = 100) -> BatchSpec:
     xy[:, 1] += np.float32(0.5 * view - 15.0 * prad + prad)
     xv = np.zeros((n, 4), np.float32)
     xv[:, :2] = xy
-    h0, h1 = wave_init_reference(1024)
+    if shallow:
+        Lw, nsub, lam, model = 128, 1, 0.001, L.CWA2D_WAVE_SHALLOW
+        h0, h1 = wave_init_shallow(Lw, view)
+    else:
+        Lw, nsub, lam, model = 1024, 10, 0.01, L.CWA2D_WAVE_LEAPFROG
+        h0, h1 = wave_init_reference(Lw)
     mass = np.float32(2 * prad) * np.float32(2 * prad) * np.float32(1000.0)
     return BatchSpec(params=p, n=np.full(nscenes, n, np.int64),
-                     wave_len=np.full(nscenes, 1024, np.int64),
-                     wave_substeps=np.full(nscenes, 10, np.int64),
+                     wave_len=np.full(nscenes, Lw, np.int64),
+                     wave_substeps=np.full(nscenes, nsub, np.int64),
                      mass=np.full(nscenes, mass, np.float32),
-                     lam=np.full(nscenes, 0.01, np.float32), xv=np.tile(xv, (nscenes, 1)),
+                     lam=np.full(nscenes, lam, np.float32), xv=np.tile(xv, (nscenes, 1)),
                      h0=np.tile(h0, nscenes), h1=np.tile(h1, nscenes), dt=0.002, steps=steps,
-                     name="SphWave2D reference scene")
+                     name="SphWave2D reference scene" + (" (Shallow1D)" if shallow else ""),
+                     wave_model=np.full(nscenes, model, np.int64),
+                     wave_dx=np.full(nscenes, 0.1, np.float32))

@@ -66,8 +66,8 @@ constexpr unsigned kRunSlotMask = (1u << kRunShift) - 1u;
 // Per-scene table (device).
 struct Scene2D {
     int pbeg, n, wbeg, L;
-    int nsub;
-    float mass, lambda, pad;
+    int nsub, model;
+    float mass, lambda, dx;
 };
 
 // Kernel-side parameters (derived constants in float, as the oracle).
@@ -131,6 +131,19 @@ __device__ __forceinline__ float wave_at(const Dev2& c, const float* __restrict_
     const int i0 = min(max((int)f, 0), L - 1);
     const int i1 = min(max((int)f + 1, 0), L - 1);
     return (1.0f - a) * w[i0] + a * w[i1];
+}
+
+// The wave's coupling velocity at x: uh/h for a shallow-water column (the
+// reference's xvel = w[1]/w[0], :362), (h^n - h^(n-1))/dt_sub for a leapfrog
+// column.
+__device__ __forceinline__ float wave_velocity(const Dev2& c, const Scene2D& S,
+                                               const float* __restrict__ w0,
+                                               const float* __restrict__ w1, float sL, float x,
+                                               float dt) {
+    if (S.model == CWA2D_WAVE_SHALLOW)
+        return wave_at(c, w1, S.L, sL, x) / wave_at(c, w0, S.L, sL, x);
+    const float dts = dt / (float)S.nsub;
+    return (wave_at(c, w0, S.L, sL, x) - wave_at(c, w1, S.L, sL, x)) / dts;
 }
 
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
@@ -316,8 +329,7 @@ __global__ void __launch_bounds__(BS) k2_density(Dev2 P, float dt, DensityArgs A
         // enable flag (:400-415); with a zero threshold an enabled particle
         // stays enabled, so the samples are skipped
         if (!(en == 1 && P.thr == 0.0f)) {
-            const float dts = dt / (float)S.nsub;
-            const float vw = (wave_at(P, w0, S.L, sL, x) - wave_at(P, w1, S.L, sL, x)) / dts;
+            const float vw = wave_velocity(P, S, w0, w1, sL, x, dt);
             if (en == 0 && fabsf(vw) > P.thr) en = 1;
             if (en == 1 && fabsf(vw) < P.thr) en = 0;
         }
@@ -446,9 +458,8 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
         float aty = 0.0f;
         if ((idf & 0x3FFFFFFF) % 5 < 4) {  // wave pressure (:586-603)
             const float sL = (float)S.L / (P.hix - P.lox);
-            const float dts = dt / (float)S.nsub;
             const float hw = wave_at(P, A.h0 + S.wbeg, S.L, sL, x);
-            const float vw = (hw - wave_at(P, A.h1 + S.wbeg, S.L, sL, x)) / dts;
+            const float vw = wave_velocity(P, S, A.h0 + S.wbeg, A.h1 + S.wbeg, sL, x, dt);
             const float dh = hw - P.wref;
             const float mask = smoothstep(0.0f, P.wmask, fabsf(vw));
             float dyw = y - hw;
@@ -475,7 +486,7 @@ __global__ void __launch_bounds__(1024)
     float* A = sm;
     float* B = sm + lds_stride;
     float* O = sm + 2 * lds_stride;
-    float* col = sm + 3 * lds_stride;
+    float* col = sm + 4 * lds_stride;
     float* g0 = h0 + S.wbeg;
     float* g1 = h1 + S.wbeg;
     for (int k = tid; k < L; k += bd) {
@@ -498,19 +509,57 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     const float lam = S.lambda;
-    const float ca = 2.0f - 2.0f * lam - P.beta, cb1 = 1.0f - P.beta;
-    for (int s = 0; s < S.nsub; ++s) {
-        for (int k = 1 + tid; k < L - 1; k += bd) {
-            const float v = P.atten * (ca * A[k] + lam * (A[k + 1] + A[k - 1]) - cb1 * B[k]);
-            O[k] = v;
-            if (k == 1) O[0] = v;          // free boundary (FreeBC :139-157)
-            if (k == L - 2) O[L - 1] = v;
+    if (S.model == CWA2D_WAVE_SHALLOW) {
+        // Shallow1D (Shallow1D_cs.glsl:133-167): A = h, B = uh, O = h at
+        // i + 1/2, M = uh at i + 1/2; each phase ends with FreeBC's copy of
+        // cells 1 and L-2 (all four channels) to the edges.
+        float* M = sm + 3 * lds_stride;
+        const float G = 9.8f, hg = 0.5f * G, l2 = lam / 2.0f, dx = S.dx;
+        auto edges = [&] {
+            if (tid == 0) {
+                A[0] = A[1], B[0] = B[1], O[0] = O[1], M[0] = M[1];
+                A[L - 1] = A[L - 2], B[L - 1] = B[L - 2], O[L - 1] = O[L - 2],
+                      M[L - 1] = M[L - 2];
+            }
+        };
+        for (int s = 0; s < S.nsub; ++s) {
+            for (int k = 1 + tid; k < L - 1; k += bd) {
+                const int e = k + 1;
+                O[k] = (A[k] + A[e]) / 2.0f - l2 * (B[e] - B[k]) / dx;
+                M[k] = (B[k] + B[e]) / 2.0f -
+                       l2 * (B[e] * B[e] / A[e] + hg * A[e] * A[e] - B[k] * B[k] / A[k] -
+                             hg * A[k] * A[k]) / dx;
+            }
+            __syncthreads();
+            edges();
+            __syncthreads();
+            for (int k = 1 + tid; k < L - 1; k += bd) {
+                const int w = k - 1;
+                const float h = A[k] - lam * (M[k] - M[w]) / dx;
+                const float uh = B[k] - lam * (M[k] * M[k] / O[k] + hg * O[k] * O[k] -
+                                               M[w] * M[w] / O[w] - hg * O[w] * O[w]) / dx;
+                A[k] = h;
+                B[k] = uh;
+            }
+            __syncthreads();
+            edges();
+            __syncthreads();
         }
-        __syncthreads();
-        float* tmp = B;
-        B = A;
-        A = O;
-        O = tmp;
+    } else {
+        const float ca = 2.0f - 2.0f * lam - P.beta, cb1 = 1.0f - P.beta;
+        for (int s = 0; s < S.nsub; ++s) {
+            for (int k = 1 + tid; k < L - 1; k += bd) {
+                const float v = P.atten * (ca * A[k] + lam * (A[k + 1] + A[k - 1]) - cb1 * B[k]);
+                O[k] = v;
+                if (k == 1) O[0] = v;          // free boundary (FreeBC :139-157)
+                if (k == L - 2) O[L - 1] = v;
+            }
+            __syncthreads();
+            float* tmp = B;
+            B = A;
+            A = O;
+            O = tmp;
+        }
     }
     if (couple) {
         const float cw = (P.hix - P.lox) / (float)L;
@@ -529,6 +578,35 @@ __global__ void __launch_bounds__(1024)
     for (int k = tid; k < L; k += bd) {
         g0[k] = A[k];
         g1[k] = B[k];
+    }
+}
+
+// Splash (Shallow1D_cs.glsl:101-115) on the shallow-water scenes (scene < 0)
+// or on one: h += -0.1 V exp(-xc^2/0.005), uh += 0.2 |dh| sign(xc).
+__global__ void __launch_bounds__(256) k2_splash(const Scene2D* __restrict__ sc, int only,
+                                                 float view_height, float* __restrict__ h,
+                                                 float* __restrict__ uh) {
+    const int s = blockIdx.y;
+    if (only >= 0 && s != only) return;
+    const Scene2D S = sc[s];
+    if (S.model != CWA2D_WAVE_SHALLOW) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < S.L; k += gridDim.x * blockDim.x) {
+        const float x = (float)k / (float)(S.L - 1);
+        const float xc = x - 0.5f;
+        const float dh = -view_height * 0.1f * expf(-xc * xc / 0.005f);
+        const float sg = xc > 0.0f ? 1.0f : (xc < 0.0f ? -1.0f : 0.0f);
+        h[S.wbeg + k] += dh;
+        uh[S.wbeg + k] += 0.2f * fabsf(dh) * sg;
+    }
+}
+
+// Render copy: (x, y) of every slot into a caller buffer.
+__global__ void __launch_bounds__(256) k2_points(const float4* __restrict__ xv, int n,
+                                                 float2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float4 p = xv[i];
+        out[i] = make_float2(p.x, p.y);
     }
 }
 
@@ -631,15 +709,20 @@ class Batch2D {
                 throw Error(CWA_EINVAL, "scene " + std::to_string(s) + ": wave_substeps in 1..1024");
             if (!(q.mass > 0.0f) || !std::isfinite(q.mass) || !std::isfinite(q.lambda))
                 throw Error(CWA_EINVAL, "scene " + std::to_string(s) + ": bad mass or lambda");
+            if (q.wave_model != CWA2D_WAVE_LEAPFROG && q.wave_model != CWA2D_WAVE_SHALLOW)
+                throw Error(CWA_EINVAL, "scene " + std::to_string(s) + ": unknown wave model");
+            if (q.wave_model == CWA2D_WAVE_SHALLOW && (!(q.wave_dx > 0.0f) || !std::isfinite(q.wave_dx)))
+                throw Error(CWA_EINVAL, "scene " + std::to_string(s) + ": wave_dx must be positive");
             Scene2D& d = host_sc_[s];
             d.pbeg = (int)po;
             d.n = q.n;
             d.wbeg = (int)wo;
             d.L = q.wave_len;
             d.nsub = q.wave_substeps;
+            d.model = q.wave_model;
             d.mass = q.mass;
             d.lambda = q.lambda;
-            d.pad = 0.0f;
+            d.dx = q.wave_dx;
             po += q.n;
             wo += q.wave_len;
             lmax_ = std::max(lmax_, q.wave_len);
@@ -651,7 +734,7 @@ class Batch2D {
         const long long cells = (long long)nscenes * dev_.ncell;
         if (cells + 1 > (1ll << 30)) throw Error(CWA_EINVAL, "too many cells in the batch");
         ncells_ = (int)cells;
-        lds_bytes_ = (size_t)(3 * lmax_ + dev_.nx) * sizeof(float);
+        lds_bytes_ = (size_t)(4 * lmax_ + dev_.nx) * sizeof(float);
         if (lds_bytes_ > 160 * 1024) throw Error(CWA_EINVAL, "wave column and grid exceed LDS");
 
         hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
@@ -760,7 +843,7 @@ class Batch2D {
         const Dev2 d = make_dev2(p);
         if (d.nx != dev_.nx || d.ny != dev_.ny)
             throw Error(CWA_EINVAL, "parameters change the grid dimensions of the batch");
-        if ((size_t)(3 * lmax_ + d.nx) * sizeof(float) > lds_bytes_)
+        if ((size_t)(4 * lmax_ + d.nx) * sizeof(float) > lds_bytes_)
             throw Error(CWA_EINVAL, "grid exceeds the wave pass's LDS");
         params_ = p;
         dev_ = d;
@@ -887,6 +970,25 @@ class Batch2D {
         hip_check(hipMemcpyAsync(&flag, finite_.get(), sizeof(int), hipMemcpyDeviceToHost, st_), "finite copy");
         hip_check(hipStreamSynchronize(st_), "sync");
         if (flag) throw Error(CWA_ENONFINITE, "non-finite particle or wave state after the step");
+    }
+
+    void Splash(int scene, float view_height) {
+        if (scene >= ns_) throw Error(CWA_EINVAL, "scene index out of range");
+        dim3 grd(blocks_for(lmax_, 256), ns_);
+        k2_splash<<<grd, 256, 0, st_>>>(sc_.get(), scene, view_height, h0_.get(), h1_.get());
+        hip_check(hipGetLastError(), "splash");
+    }
+
+    void Render(float* points_xy, float* wave) {
+        if (points_xy) {
+            k2_points<<<blocks_for(ntot_, 256), 256, 0, st_>>>(
+                xv_[0].get(), ntot_, reinterpret_cast<float2*>(points_xy));
+            hip_check(hipGetLastError(), "render points");
+        }
+        if (wave)
+            hip_check(hipMemcpyAsync(wave, h0_.get(), sizeof(float) * wtot_,
+                                     hipMemcpyDeviceToDevice, st_),
+                      "render wave");
     }
 
     void Sync() { hip_check(hipStreamSynchronize(st_), "sync"); }
@@ -1247,6 +1349,29 @@ int cwa2d_wave_init_reference(int32_t L, float* h0, float* h1) {
             }
         }
     });
+}
+
+int cwa2d_wave_init_shallow(int32_t L, float view_height, float* h, float* uh) {
+    return guarded2([&] {
+        if (L < 3 || !h || !uh) throw Error(CWA_EINVAL, "wave_init_shallow: L >= 3 and two outputs");
+        for (int k = 0; k < L; ++k) {
+            const float x = (float)k / (float)(L - 1);
+            const float xc = x - 0.5f;
+            const float h_free = view_height * 0.5f;
+            const float bump = view_height * 0.1f * std::exp(-xc * xc / 0.005f);
+            const float sg = xc > 0.0f ? 1.0f : (xc < 0.0f ? -1.0f : 0.0f);
+            h[k] = h_free + bump;
+            uh[k] = 0.5f * std::fabs(bump) * sg;
+        }
+    });
+}
+
+int cwa2d_batch_splash(cwa2d_batch* b, int32_t scene, float view_height) {
+    return guarded2([&] { impl(b)->Splash(scene, view_height); });
+}
+
+int cwa2d_batch_render(cwa2d_batch* b, float* points_xy, float* wave) {
+    return guarded2([&] { impl(b)->Render(points_xy, wave); });
 }
 
 int cwa2d_batch_create(const cwa2d_params* p, int32_t nscenes, const cwa2d_scene* scenes,

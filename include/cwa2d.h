@@ -17,6 +17,9 @@
  *                        ImageStencil::GetReadImage
  *   cwa2d_wave_init_reference
  *                        Wave1D_cs.glsl InitWave :96-118 (modes 0 and 1)
+ *   wave_model = CWA2D_WAVE_SHALLOW
+ *                        Shallow1D_cs.glsl IterateWave :133-167, InitWave :117-131,
+ *                        Splash :101-115 (the wave SphWave2D/Main.cpp:283 runs)
  * Status codes and cwa_last_error() are those of cwa.h.
  */
 #ifndef CWA2D_H
@@ -31,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CWA2D_ABI_VERSION 1
+#define CWA2D_ABI_VERSION 2
 #define CWA2D_MAX_WAVE_LEN 8192      /* cells of one wave column (LDS-resident) */
 #define CWA2D_MAX_PARTICLES (1 << 26) /* slots of the whole batch */
 
@@ -63,13 +66,22 @@ typedef struct cwa2d_params {
     int32_t check_finite;  /* nonzero: every step checks the state for NaN/Inf   */
 } cwa2d_params;
 
+/* Wave models of a scene's 1D column. */
+#define CWA2D_WAVE_LEAPFROG 0 /* Wave1D_cs.glsl: levels h0 = h^n, h1 = h^(n-1)       */
+#define CWA2D_WAVE_SHALLOW 1  /* Shallow1D_cs.glsl: h0 = water height h, h1 = uh     */
+
 /* One scene of the batch. */
 typedef struct cwa2d_scene {
     int32_t n;             /* particles (>= 1)                                   */
     int32_t wave_len;      /* wave cells, 3..CWA2D_MAX_WAVE_LEN                  */
     int32_t wave_substeps; /* ImageStencil::SetSubsteps (>= 1)                   */
+    int32_t wave_model;    /* CWA2D_WAVE_LEAPFROG or CWA2D_WAVE_SHALLOW          */
     float mass;            /* MASS (:124)                                        */
-    float lambda;          /* (c dt_sub / dx)^2, Wave1D_cs.glsl:18               */
+    float lambda;          /* leapfrog: (c dt_sub / dx)^2 (Wave1D_cs.glsl:18);
+                              shallow: the Lax-Wendroff time step
+                              (Shallow1D_cs.glsl:22)                              */
+    float wave_dx;         /* shallow: the scheme's cell size dx
+                              (Shallow1D_cs.glsl:23); unused by the leapfrog     */
 } cwa2d_scene;
 
 /* Launch configuration (the autotuner's output). */
@@ -124,6 +136,10 @@ int cwa2d_params_reference(cwa2d_params* p);
  * mode 0 then mode 1, so h1 (older) has the bump at int(0.25 L) and h0
  * (newer) one cell further.  Host computation. */
 int cwa2d_wave_init_reference(int32_t L, float* h0, float* h1);
+/* Shallow1D InitWave (Shallow1D_cs.glsl:117-131): h = view_height/2 +
+ * 0.1 view_height exp(-xc^2/0.005), uh = 0.5 |bump| sign(xc), xc = k/(L-1) - 1/2.
+ * Host computation. */
+int cwa2d_wave_init_shallow(int32_t L, float view_height, float* h, float* uh);
 
 /* xv: 4 floats per particle (x, y, vx, vy), scenes concatenated in order;
  * enabled: per particle 0/1 or null (all enabled, as InitParticle :375);
@@ -146,6 +162,17 @@ int cwa2d_batch_step_graph(cwa2d_batch* b, float dt, int32_t steps);
 /* One step with a HIP event pair around every kernel. */
 int cwa2d_batch_step_profiled(cwa2d_batch* b, float dt, cwa2d_kernel_times* t);
 int cwa2d_batch_sync(cwa2d_batch* b);
+/* The "Splash" button (SphWave2D/Main.cpp:124-129, ImageStencil::ComputeFunc(1)):
+ * Shallow1D's Splash (Shallow1D_cs.glsl:101-115) on every shallow-water scene
+ * (scene < 0) or on one; view_height scales the bump.  Leapfrog scenes are
+ * left unchanged. */
+int cwa2d_batch_splash(cwa2d_batch* b, int32_t scene, float view_height);
+/* Render interop: copies the particle positions (float2 per slot, slot
+ * order) and the newest wave level into caller device memory, e.g. the
+ * pointers of GL buffers mapped with hipGraphicsGLRegisterBuffer /
+ * hipGraphicsResourceGetMappedPointer.  Either pointer may be null.
+ * Ordered on the batch's stream. */
+int cwa2d_batch_render(cwa2d_batch* b, float* points_xy, float* wave);
 int cwa2d_batch_stream(cwa2d_batch* b, void** hip_stream);
 int cwa2d_batch_views(cwa2d_batch* b, cwa2d_views* v);
 

@@ -87,8 +87,10 @@ BatchBase* make(const cwa2d_params* p, int nscenes, const cwa2d_scene* sc, const
         A.n = sc[s].n;
         A.L = sc[s].wave_len;
         A.nsub = std::max(1, (int)sc[s].wave_substeps);
+        A.model = sc[s].wave_model;
         A.mass = R(sc[s].mass);
         A.lambda = R(sc[s].lambda);
+        A.dx = R(sc[s].wave_dx);
         A.x.resize(A.n);
         A.y.resize(A.n);
         A.vx.resize(A.n);
@@ -130,7 +132,9 @@ void* cwao2_create(const cwa2d_params* p, int nscenes, const cwa2d_scene* sc, co
                    const int32_t* en, const float* h0, const float* h1, int use_double) {
     if (!p || nscenes <= 0 || !sc || !xv) return nullptr;
     for (int s = 0; s < nscenes; ++s)
-        if (sc[s].n < 1 || sc[s].wave_len < 3) return nullptr;
+        if (sc[s].n < 1 || sc[s].wave_len < 3 ||
+            (sc[s].wave_model != CWA2D_WAVE_LEAPFROG && sc[s].wave_model != CWA2D_WAVE_SHALLOW))
+            return nullptr;
     if (use_double) return make<double>(p, nscenes, sc, xv, en, h0, h1);
     return make<float>(p, nscenes, sc, xv, en, h0, h1);
 }
@@ -182,5 +186,13 @@ double cwao2_wave_at_d(const cwa2d_params* p, const double* w, int L, double x) 
     return cwao2::wave_at(c, w, L, x);
 }
 float cwao2_column_sum_f(const float* f, int n) { return cwao2::column_sum(f, 0, n); }
+
+// nsub shallow-water substeps of (h, uh) in double, in place.
+void cwao2_shallow_d(double* h, double* uh, int L, double lam, double dx, int nsub) {
+    std::vector<double> H(h, h + L), U(uh, uh + L), Hm(L, 0.0), Um(L, 0.0);
+    for (int s = 0; s < nsub; ++s) cwao2::shallow_substep(H, U, Hm, Um, lam, dx);
+    std::copy(H.begin(), H.end(), h);
+    std::copy(U.begin(), U.end(), uh);
+}
 
 }  // extern "C"

@@ -9,6 +9,8 @@
 //     boundary_sdf :205-230 (opU keeps the later plane on ties, :191-194)
 //     ComputeDensityPressure :393-487, ComputeForces :490-628
 //   SphWave2D/Wave1D_cs.glsl IterateWave :120-130, FreeBC :139-157
+//   SphWave2D/Shallow1D_cs.glsl IterateWave :133-167 (two-phase Lax-Wendroff,
+//     FreeBC copies the full cell state to the edges after each phase)
 //   SphWave2D/StencilBuffer.cpp:150-179 (SphUgrid::Compute: grid rebuilt
 //     every SPH substep), StencilImage2D.cpp:141-164 (wave substeps)
 //   UniformGrid2D/UniformGrid2D.cpp:33-92 (counting sort, stable by index)
@@ -25,10 +27,10 @@
 //   r*r in viscosity   -> the same fma chain r2
 //   texture(1D, LINEAR, CLAMP) -> fp32 weights, t = u*L - 1/2
 // Additions the reference leaves undefined (recorded as decisions):
-//   * The coupling velocity of the enable flag and the wave pressure is the
-//     wave's vertical velocity (h^n - h^(n-1)) / dt_sub at x; the reference
-//     divides Shallow1D's momentum by height (xvel = w[1]/w[0]), which is
-//     not a velocity for Wave1D.
+//   * The coupling velocity of the enable flag and the wave pressure: for a
+//     shallow-water column the reference's own xvel = w[1]/w[0] = uh/h
+//     (:362); for a leapfrog column, whose second channel is no momentum,
+//     the wave's vertical velocity (h^n - h^(n-1)) / dt_sub at x.
 //   * Particle -> wave flux (couple_band > 0): (m/rho0) v_y (1 - q^2)^2 of
 //     particles within the band of the surface, summed per grid column in a
 //     fixed order (64 lane-strided partial sums, then a xor butterfly) and
@@ -169,12 +171,13 @@ static inline int cell_axis2(R p, R lo, R H, int dim) {
 template <class R>
 struct Scene2 {
     int n = 0, L = 0, nsub = 1;
-    R mass = 0, lambda = 0;
+    int model = CWA2D_WAVE_LEAPFROG;
+    R mass = 0, lambda = 0, dx = 1;
     // slot-ordered state (slots are cell-sorted after each grid build)
     std::vector<R> x, y, vx, vy, rho, prs;
     std::vector<int32_t> id;   // scene-local id
     std::vector<int32_t> en;   // enabled flag
-    std::vector<R> h0, h1;     // newest / previous wave level
+    std::vector<R> h0, h1;     // leapfrog: newest / previous level; shallow: h, uh
     // last grid build: local cell of each (pre-sort) slot, offsets
     std::vector<int32_t> key, offset;
     std::vector<R> colsum;
@@ -243,6 +246,46 @@ static inline void for_candidates(const K2<R>& c, const Scene2<R>& S, R px, R py
     }
 }
 
+// The wave's coupling velocity at x (see the header: uh/h for shallow
+// water, (h^n - h^(n-1))/dt_sub for the leapfrog).
+template <class R>
+static inline R wave_velocity(const K2<R>& c, const Scene2<R>& S, R x, R dts) {
+    if (S.model == CWA2D_WAVE_SHALLOW)
+        return wave_at(c, S.h1.data(), S.L, x) / wave_at(c, S.h0.data(), S.L, x);
+    return (wave_at(c, S.h0.data(), S.L, x) - wave_at(c, S.h1.data(), S.L, x)) / dts;
+}
+
+// Shallow1D substep (Shallow1D_cs.glsl:133-167): the half-step values at
+// i + 1/2, the full step, each phase followed by FreeBC's copy of cells
+// 1 and L-2 (all four channels) to the edges.
+template <class R>
+static void shallow_substep(std::vector<R>& H, std::vector<R>& U, std::vector<R>& Hm,
+                            std::vector<R>& Um, R lam, R dx) {
+    const int L = (int)H.size();
+    const R G = R(9.8f), hg = R(0.5) * G, l2 = lam / R(2);
+    for (int k = 1; k < L - 1; ++k) {
+        const int e = k + 1;
+        Hm[k] = (H[k] + H[e]) / R(2) - l2 * (U[e] - U[k]) / dx;
+        Um[k] = (U[k] + U[e]) / R(2) -
+                l2 * (U[e] * U[e] / H[e] + hg * H[e] * H[e] - U[k] * U[k] / H[k] -
+                      hg * H[k] * H[k]) / dx;
+    }
+    auto edges = [&] {
+        H[0] = H[1], U[0] = U[1], Hm[0] = Hm[1], Um[0] = Um[1];
+        H[L - 1] = H[L - 2], U[L - 1] = U[L - 2], Hm[L - 1] = Hm[L - 2], Um[L - 1] = Um[L - 2];
+    };
+    edges();
+    for (int k = 1; k < L - 1; ++k) {
+        const int w = k - 1;
+        const R h = H[k] - lam * (Um[k] - Um[w]) / dx;
+        const R uh = U[k] - lam * (Um[k] * Um[k] / Hm[k] + hg * Hm[k] * Hm[k] -
+                                   Um[w] * Um[w] / Hm[w] - hg * Hm[w] * Hm[w]) / dx;
+        H[k] = h;
+        U[k] = uh;
+    }
+    edges();
+}
+
 // One coupled step of one scene: sph_substeps x (grid, density, forces),
 // then the wave substeps and the flux source.
 template <class R>
@@ -259,7 +302,7 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
         for (int i = 0; i < n; ++i) {
             R x = S.x[i], y = S.y[i], vx = S.vx[i], vy = S.vy[i];
             {   // enable flag from the wave velocity at the raw x (:400-415)
-                const R vw = (wave_at(c, S.h0.data(), L, x) - wave_at(c, S.h1.data(), L, x)) / dts;
+                const R vw = wave_velocity(c, S, x, dts);
                 if (S.en[i] == 0 && std::fabs(vw) > c.thr) S.en[i] = 1;
                 if (S.en[i] == 1 && std::fabs(vw) < c.thr) S.en[i] = 0;
             }
@@ -350,7 +393,7 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
             R aty = R(0);
             if (S.id[i] % 5 < 4) {  // wave pressure (:586-603)
                 const R hw = wave_at(c, S.h0.data(), L, x);
-                const R vw = (hw - wave_at(c, S.h1.data(), L, x)) / dts;
+                const R vw = wave_velocity(c, S, x, dts);
                 const R dh = hw - c.wref;
                 const R mask = smoothstep(R(0), c.wmask, std::fabs(vw));
                 R dyw = y - hw;
@@ -375,17 +418,23 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
     if (c.band > R(0))
         for (int cx = 0; cx < c.nx; ++cx)
             S.colsum[cx] = column_sum(flux.data(), S.offset[cx * c.ny], S.offset[(cx + 1) * c.ny]);
-    // ---- wave substeps, free boundary (Wave1D_cs.glsl:120-157) ----
+    // ---- wave substeps, free boundary (Wave1D_cs.glsl:120-157 or
+    //      Shallow1D_cs.glsl:133-167) ----
     std::vector<R> a(S.h0), b(S.h1), o(L);
     const R lam = S.lambda;
-    const R ca = R(2) - R(2) * lam - c.beta, cb = R(1) - c.beta;
-    for (int s = 0; s < S.nsub; ++s) {
-        for (int k = 1; k < L - 1; ++k)
-            o[k] = c.atten * (ca * a[k] + lam * (a[k + 1] + a[k - 1]) - cb * b[k]);
-        o[0] = o[1];
-        o[L - 1] = o[L - 2];
-        b.swap(a);  // b <- a
-        a.swap(o);  // a <- o
+    if (S.model == CWA2D_WAVE_SHALLOW) {
+        std::vector<R> hm(L, R(0)), um(L, R(0));
+        for (int s = 0; s < S.nsub; ++s) shallow_substep(a, b, hm, um, lam, S.dx);
+    } else {
+        const R ca = R(2) - R(2) * lam - c.beta, cb = R(1) - c.beta;
+        for (int s = 0; s < S.nsub; ++s) {
+            for (int k = 1; k < L - 1; ++k)
+                o[k] = c.atten * (ca * a[k] + lam * (a[k + 1] + a[k - 1]) - cb * b[k]);
+            o[0] = o[1];
+            o[L - 1] = o[L - 2];
+            b.swap(a);  // b <- a
+            a.swap(o);  // a <- o
+        }
     }
     if (c.band > R(0)) {
         const R cw = (c.hix - c.lox) / R(L);

@@ -43,6 +43,7 @@ def lib():
             "cwao2_sdf_d": (C.c_double, [P, C.c_double, C.c_double, dp]),
             "cwao2_wave_at_d": (C.c_double, [P, dp, C.c_int, C.c_double]),
             "cwao2_column_sum_f": (C.c_float, [fp, C.c_int]),
+            "cwao2_shallow_d": (None, [dp, dp, C.c_int, C.c_double, C.c_double, C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -84,6 +85,15 @@ def wave_at(params, w, x: float) -> float:
 def column_sum(f) -> float:
     f = np.ascontiguousarray(f, np.float32)
     return lib().cwao2_column_sum_f(_p(f, C.c_float), int(f.size))
+
+
+def shallow(h, uh, lam: float, dx: float, nsub: int):
+    """nsub Shallow1D substeps of (h, uh) in double."""
+    h = np.array(h, np.float64)
+    uh = np.array(uh, np.float64)
+    lib().cwao2_shallow_d(_p(h, C.c_double), _p(uh, C.c_double), int(h.size), float(lam),
+                          float(dx), int(nsub))
+    return h, uh
 
 
 class OracleBatch:

@@ -36,7 +36,7 @@ def test_2d_struct_sizes_and_rejections(cwa):
     from coupledwateranimation_amd import _lib, batch2d
 
     L = cwa.lib()
-    assert L.cwa2d_abi_version() == 1
+    assert L.cwa2d_abi_version() == 2
     for k, st in enumerate(_lib.STRUCTS_2D):
         assert L.cwa2d_struct_size(k) == C.sizeof(st)
     assert L.cwa2d_struct_size(9) == -1

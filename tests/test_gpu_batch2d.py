@@ -208,3 +208,73 @@ def test_c3_batch_invariants_over_500_steps(gpu, cwa):
     rho = g["rp"][:, 0]
     assert rho.min() >= 1000.0 and rho.max() < 1e5
     b.close()
+
+
+# ------------------------------------------------------------ Shallow1D
+def test_shallow_reference_scene_bit_exact(gpu, cwa, o2):
+    """The wave the application runs: Shallow1D, 128 cells, one substep,
+    lambda = 0.001, dx = 0.1; coupling velocity uh/h."""
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.reference_batch(2, shallow=True)
+    b, g, o = _run_both(spec, 30, o2=o2)
+    _assert_slots_equal(g, o.slots(), "shallow reference scene, 30 steps")
+    b.close()
+
+
+def _mixed_spec():
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.small_batch(nscenes=6, seed=13)
+    models = np.array([0, 1, 0, 1, 1, 0])
+    spec.wave_model = models
+    spec.wave_dx = (1.0 / spec.wave_len).astype(np.float32)
+    wb = spec.wbeg()
+    for s in np.flatnonzero(models == 1):
+        Lw = int(spec.wave_len[s])
+        k = np.arange(Lw)
+        spec.h0[wb[s]:wb[s] + Lw] = 0.3 + 0.02 * np.exp(-((k - Lw / 3) / (Lw / 20)) ** 2)
+        spec.h1[wb[s]:wb[s] + Lw] = 0.0
+        spec.lam[s] = spec.dt / spec.wave_substeps[s]
+    return spec
+
+
+def test_mixed_wave_models_bit_exact(gpu, cwa, o2):
+    spec = _mixed_spec()
+    b, g, o = _run_both(spec, 25, graph=True, o2=o2)
+    _assert_slots_equal(g, o.slots(), "mixed leapfrog / shallow batch")
+    b.close()
+
+
+def test_splash_and_render(gpu, cwa, o2):
+    import torch
+
+    from coupledwateranimation_amd import batch2d
+
+    spec = batch2d.reference_batch(2, shallow=True)
+    b = spec.make()
+    b.splash(-1, 9.6)
+    g = b.slots()
+    k = np.arange(128)
+    xc = (k / 127.0 - 0.5).astype(np.float32)
+    dh = -9.6 * 0.1 * np.exp(-xc.astype(np.float64) ** 2 / 0.005)
+    exp_h = np.tile(spec.h0.astype(np.float64)[:128] + dh, 2)
+    exp_u = np.tile(spec.h1.astype(np.float64)[:128] + 0.2 * np.abs(dh) * np.sign(xc), 2)
+    np.testing.assert_allclose(g["h0"], exp_h, rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(g["h1"], exp_u, rtol=2e-5, atol=2e-6)
+    # the splashed state steps like the oracle started from it
+    spec2 = batch2d.reference_batch(2, shallow=True)
+    spec2.h0, spec2.h1 = g["h0"].copy(), g["h1"].copy()
+    b.step(spec.dt, 10)
+    o = o2.OracleBatch(spec2)
+    o.step(spec.dt, 10)
+    _assert_slots_equal(b.slots(), o.slots(), "after splash")
+    # render copies into caller device memory
+    pts = torch.zeros((b.n_total, 2), dtype=torch.float32, device="cuda")
+    wav = torch.zeros(int(spec.wave_len.sum()), dtype=torch.float32, device="cuda")
+    b.render(pts.data_ptr(), wav.data_ptr())
+    b.sync()
+    g = b.slots()
+    assert np.array_equal(pts.cpu().numpy(), g["xv"][:, :2])
+    assert np.array_equal(wav.cpu().numpy(), g["h0"])
+    b.close()

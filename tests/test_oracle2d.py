@@ -328,3 +328,55 @@ def test_c3_workload_shape():
         blk = spec.xv[b[s]:b[s] + spec.n[s]]
         assert blk[:, 0].min() > spec.params.walls[1] and blk[:, 0].max() < 0.52
         assert blk[:, 1].min() > spec.params.walls[0]
+
+
+# ------------------------------------------------------------ Shallow1D
+def test_shallow_lake_at_rest_is_exact(o2):
+    """Flat water at rest: every flux difference vanishes exactly."""
+    h, uh = o2.shallow(np.full(64, 4.8), np.zeros(64), lam=0.001, dx=0.1, nsub=50)
+    assert np.all(h == 4.8) and np.all(uh == 0.0)
+
+
+def test_shallow_mass_telescopes(o2):
+    """The full step changes h by -lam (F_{k+1/2} - F_{k-1/2}) / dx, so the
+    interior mass changes only through the edge half-fluxes; with the pulse
+    far from the edges, sum h is conserved to rounding."""
+    L_ = 400
+    x = np.arange(L_)
+    h = 1.0 + 0.05 * np.exp(-((x - 200.0) / 10.0) ** 2)
+    uh = np.zeros(L_)
+    h2, uh2 = o2.shallow(h, uh, lam=0.002, dx=0.1, nsub=40)
+    assert abs(h2.sum() - h.sum()) < 1e-10 * h.sum()
+    assert abs(uh2.sum()) < 1e-9                      # symmetric split: no net momentum
+
+
+def test_shallow_small_pulse_moves_at_sqrt_gh(o2):
+    """A small right-moving pulse (uh = c eta, c = sqrt(g h0)) travels at c."""
+    L_, dx, lam, h0 = 2000, 0.1, 0.002, 1.0
+    g = float(np.float32(9.8))
+    c = math.sqrt(g * h0)
+    x = np.arange(L_) * dx
+    eta = 1e-4 * np.exp(-((x - 40.0) / 1.5) ** 2)
+    h, uh = o2.shallow(h0 + eta, c * eta, lam=lam, dx=dx, nsub=2000)
+    xpk = x[np.argmax(h)]
+    assert xpk == pytest.approx(40.0 + c * lam * 2000, abs=3 * dx)
+
+
+def test_shallow_init_matches_reference_formula():
+    h, uh = batch2d.wave_init_shallow(128, 9.6)
+    k = np.arange(128)
+    xc = k / 127.0 - 0.5
+    bump = 9.6 * 0.1 * np.exp(-xc * xc / 0.005)
+    np.testing.assert_allclose(h, 4.8 + bump, rtol=1e-6)
+    np.testing.assert_allclose(uh, 0.5 * bump * np.sign(xc), rtol=1e-5, atol=1e-7)
+
+
+def test_shallow_reference_scene_runs_and_couples(o2):
+    """The application's Shallow1D scene: finite, the wave moves, and the
+    coupling velocity uh/h enables particles under the moving water."""
+    spec = batch2d.reference_batch(1, shallow=True)
+    o = o2.OracleBatch(spec)
+    o.step(spec.dt, 20)
+    st = o.state()
+    assert np.isfinite(st["xv"]).all() and np.isfinite(st["h0"]).all()
+    assert np.abs(st["h0"] - spec.h0).max() > 1e-3
