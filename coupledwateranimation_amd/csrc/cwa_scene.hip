@@ -142,10 +142,14 @@ void HeightField2DTripleBuffered::Upload(const float* h0, const float* h1, hipSt
 
 void HeightField2DTripleBuffered::ReinitFromTexture(const float* tex, int tex_w, int tex_h,
                                                     hipStream_t st) {
-    if (tex_w < 2 * w_ || tex_h < h_) throw Error(CWA_EINVAL, "init texture too small");
-    std::vector<float> f((size_t)w_ * h_);
-    for (int j = 0; j < h_; ++j)
-        for (int i = 0; i < w_; ++i) f[(size_t)j * w_ + i] = tex[(size_t)j * tex_w + 2 * i];
+    if (tex_w < 1 || tex_h < 1 || !tex) throw Error(CWA_EINVAL, "empty init texture");
+    // texelFetch(uInitImage, (2x, y)) (wave_comp.glsl:76-80); a fetch outside
+    // the texture reads 0, the robust-access result (GL leaves it undefined,
+    // and the reference's 64^2 textures on its 64^2 field fetch past x = 31)
+    std::vector<float> f((size_t)w_ * h_, 0.0f);
+    for (int j = 0; j < h_ && j < tex_h; ++j)
+        for (int i = 0; i < w_ && 2 * i < tex_w; ++i)
+            f[(size_t)j * w_ + i] = tex[(size_t)j * tex_w + 2 * i];
     hip_check(hipMemcpyAsync(GetWriteImage(), f.data(), f.size() * sizeof(float),
                              hipMemcpyHostToDevice, st),
               "texture upload");

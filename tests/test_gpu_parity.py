@@ -365,3 +365,25 @@ def test_large_scene_step(gpu, cwa, oracle):
     lo, hi = np.array(spec.params.lower), np.array(spec.params.upper)
     assert np.all(st["pos"][:, :3] >= lo) and np.all(st["pos"][:, :3] <= hi)
     s.close()
+
+
+def test_init_from_reference_textures(gpu, cwa):
+    """InitFromImage with the reference's init textures on its 64 x 64
+    field: texel (2x, y) of the red channel / 255, fetches past the
+    texture's width read 0."""
+    from pathlib import Path
+
+    from coupledwateranimation_amd import configs, files
+
+    gold = np.load(Path(__file__).with_name("golden") / "init_textures.npz", allow_pickle=False)
+    spec = configs.reference_scene()
+    with spec.make_scene() as s:
+        for name in ("breaking_wave", "splash", "wake"):
+            tex = files.texture_from_pixels(gold[name])
+            prev = s.readback()["height0"].copy()
+            s.wave_reinit_from_texture(tex)
+            st = s.readback()
+            exp = np.zeros((64, 64), np.float32)
+            exp[:, :32] = tex[:, ::2]
+            assert np.array_equal(st["height0"], exp), name
+            assert np.array_equal(st["height1"], prev), name
