@@ -41,7 +41,8 @@ class Params(C.Structure):
         ("lambda_z", C.c_float), ("atten", C.c_float), ("beta", C.c_float),
         ("wave_type", C.c_float), ("wake_threshold", C.c_float), ("wake_amount", C.c_float),
         ("couple_band", C.c_float), ("couple_strength", C.c_float), ("grid_min", f3),
-        ("cell_size", C.c_float), ("grid_dims", i3), ("check_finite", C.c_int32),
+        ("cell_size", C.c_float), ("grid_dims", i3), ("grid_ydiv", C.c_int32),
+        ("check_finite", C.c_int32),
     ]
 
     def copy(self) -> "Params":

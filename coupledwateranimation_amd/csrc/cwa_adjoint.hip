@@ -99,18 +99,6 @@ __device__ __forceinline__ bool coupled_(const DevParams& P, float y, float H) {
     return !(P.band > 0.0f) || fabsf(y - H) < P.band;
 }
 
-__device__ __forceinline__ void column_range_(const DevParams& P, const int* __restrict__ offset,
-                                              int ix, int iy, int iz, int c, int& b, int& e) {
-    const int cx = ix + c / 3 - 1, cz = iz + c % 3 - 1;
-    if (cx < 0 || cx >= P.dims[0] || cz < 0 || cz >= P.dims[2]) {
-        b = e = 0;
-        return;
-    }
-    const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.dims[1] - 1);
-    b = offset[lin_cell(P.dims, cx, ylo, cz)];
-    e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
-}
-
 // Visit the frozen neighbours of particle i (bit order), excluding i.
 template <class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
@@ -119,12 +107,11 @@ __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
                                                    const float4* __restrict__ pos, int n, int i,
                                                    float4 pi, F&& f) {
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
-    const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
-        column_range_(P, offset, ix, iy, iz, c, b, e);
+        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
         const uint2 mk = nbmask[(size_t)c * n + i];
         unsigned long long m = ((unsigned long long)mk.y << 32) | mk.x;
         while (m) {

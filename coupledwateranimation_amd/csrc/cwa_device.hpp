@@ -44,7 +44,9 @@ struct DevParams {
     float ax, bx, az, bz;  // texel -> column-grid coordinate maps
     // grid
     float gmin[3];
-    float cell;
+    float cell;      // x and z cell edge
+    float cell_y;    // y cell edge (cell / grid_ydiv)
+    float grid_eps;  // absolute margin of the trimmed candidate rows
     int dims[3];
     int num_cells;
     // Device copy of this block.  Kernels read their parameters through it,
@@ -71,6 +73,39 @@ __device__ __forceinline__ int cell_axis(float p, float gmin, float cell, int di
 
 __device__ __forceinline__ int lin_cell(const int dims[3], int ix, int iy, int iz) {
     return (ix * dims[2] + iz) * dims[1] + iy;
+}
+
+// Candidate slot range of a query point p in column c of its 3 x 3 column
+// neighbourhood (cx = ix + c/3 - 1, cz = iz + c%3 - 1): the rows of that
+// column the support sphere can reach.  The column is skipped when its
+// footprint lies at horizontal distance >= h, and its rows are cut to
+// |y - py| < sqrt(h^2 - d^2) with d the horizontal distance to the
+// footprint; both bounds are widened by grid_eps (well above the rounding
+// of the float bounds), so every neighbour stays inside and the exact
+// per-candidate test decides.  Density, force and the adjoint call this with
+// the same positions, so their candidate ranges (and mask bits) agree.
+__device__ __forceinline__ void column_range(const DevParams& P, const int* __restrict__ offset,
+                                             float px, float py, float pz, int ix, int iz, int c,
+                                             int& b, int& e) {
+    const int cx = ix + c / 3 - 1, cz = iz + c % 3 - 1;
+    b = e = 0;
+    if (cx < 0 || cx >= P.dims[0] || cz < 0 || cz >= P.dims[2]) return;
+    float dx = 0.0f, dz = 0.0f;
+    if (cx < ix) dx = px - (P.gmin[0] + (float)(cx + 1) * P.cell);
+    else if (cx > ix) dx = (P.gmin[0] + (float)cx * P.cell) - px;
+    if (cz < iz) dz = pz - (P.gmin[2] + (float)(cz + 1) * P.cell);
+    else if (cz > iz) dz = (P.gmin[2] + (float)cz * P.cell) - pz;
+    dx = fmaxf(dx - P.grid_eps, 0.0f);
+    dz = fmaxf(dz - P.grid_eps, 0.0f);
+    const float d2 = dx * dx + dz * dz;
+    if (!(d2 < P.r2_lt)) return;
+    // the hardware square root (about 1 ulp) is enough for a bound that
+    // carries the grid_eps margin, and it is the same in every kernel
+    const float ry = __builtin_amdgcn_sqrtf(P.r2_lt - d2) + P.grid_eps;
+    const int ylo = cell_axis(py - ry, P.gmin[1], P.cell_y, P.dims[1]);
+    const int yhi = cell_axis(py + ry, P.gmin[1], P.cell_y, P.dims[1]);
+    b = offset[lin_cell(P.dims, cx, ylo, cz)];
+    e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
 }
 
 // ---------------------------------------------------------------------------

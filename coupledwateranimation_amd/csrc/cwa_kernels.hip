@@ -59,6 +59,38 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+// The kScanItems (8) consecutive values of a thread: two 16-byte loads when
+// the whole chunk is in range and the array is 16-byte aligned (a chunk
+// starts at a multiple of 8 ints), element loads otherwise.
+__device__ __forceinline__ bool aligned16(const void* p) {
+    return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+__device__ __forceinline__ void load_items(const int* __restrict__ in, int base, int n,
+                                           int v[kScanItems]) {
+    if (base + kScanItems <= n && aligned16(in)) {
+        const int4 a = *reinterpret_cast<const int4*>(in + base);
+        const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+        v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = (base + k < n) ? in[base + k] : 0;
+    }
+}
+
+__device__ __forceinline__ void store_items(int* __restrict__ out, int base, int n,
+                                            const int v[kScanItems]) {
+    if (base + kScanItems <= n && aligned16(out)) {
+        *reinterpret_cast<int4*>(out + base) = make_int4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<int4*>(out + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n) out[base + k] = v[k];
+    }
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -91,35 +123,33 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int* __restri
                                                              int* __restrict__ zero_out) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
+    load_items(in, base, n, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
-        v[k] = (i < n) ? in[i] : 0;
-        s += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     int total;
     int run = block_excl_scan(s, total) + tile_prefix[blockIdx.x];
+    int o[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
-        if (i < n) {
-            out[i] = run;
-            if (zero_out) zero_out[i] = 0;
-        }
+        o[k] = run;
         run += v[k];
+    }
+    store_items(out, base, n, o);
+    if (zero_out) {
+        const int z[kScanItems] = {0, 0, 0, 0, 0, 0, 0, 0};
+        store_items(zero_out, base, n, z);
     }
 }
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int* __restrict__ in, int n,
                                                               int* __restrict__ part) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    load_items(in, base, n, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
-        s += (i < n) ? in[i] : 0;
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     __shared__ int wsum[kScanThreads / 64];
     s = wave_sum_i32(s);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
@@ -151,23 +181,22 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int* __restric
     }
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
+    load_items(in, base, n, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
-        v[k] = (i < n) ? in[i] : 0;
-        s += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     int total;
     int run = block_excl_scan(s, total) + prefix;
+    int o[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
-        if (i < n) {
-            out[i] = run;
-            if (zero_out) zero_out[i] = 0;
-        }
+        o[k] = run;
         run += v[k];
+    }
+    store_items(out, base, n, o);
+    if (zero_out) {
+        const int z[kScanItems] = {0, 0, 0, 0, 0, 0, 0, 0};
+        store_items(zero_out, base, n, z);
     }
     if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = total + prefix;
 }
@@ -233,7 +262,7 @@ __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ p
     if (valid) {
         const float4 p = pos[i];
         const int ix = cell_axis(p.x, g.gmin[0], g.cell, g.dims[0]);
-        const int iy = cell_axis(p.y, g.gmin[1], g.cell, g.dims[1]);
+        const int iy = cell_axis(p.y, g.gmin[1], g.cell_y, g.dims[1]);
         const int iz = cell_axis(p.z, g.gmin[2], g.cell, g.dims[2]);
         c = lin_cell(g.dims, ix, iy, iz);
         key[i] = c;
@@ -298,18 +327,6 @@ __global__ void k_stable_reorder(const int* __restrict__ key, const int* __restr
 // c%3 - 1) (dx outer, dz inner, the oracle's order); its candidates are the
 // contiguous sorted range of cells iy-1..iy+1 of that column.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void column_range(const DevParams& P, const int* __restrict__ offset,
-                                             int ix, int iy, int iz, int c, int& b, int& e) {
-    const int cx = ix + c / 3 - 1, cz = iz + c % 3 - 1;
-    if (cx < 0 || cx >= P.dims[0] || cz < 0 || cz >= P.dims[2]) {
-        b = e = 0;
-        return;
-    }
-    const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.dims[1] - 1);
-    b = offset[lin_cell(P.dims, cx, ylo, cz)];
-    e = offset[lin_cell(P.dims, cx, yhi, cz) + 1];
-}
-
 __device__ __forceinline__ bool coupled(const DevParams& P, float y, float H) {
     return !(P.band > 0.0f) || fabsf(y - H) < P.band;
 }
@@ -348,14 +365,13 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
-    const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
-        column_range(P, offset, ix, iy, iz, c, b, e);
+        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
         const float4* __restrict__ q = pos + b;
         const int len = e - b;
         uint32_t lo = 0, hi = 0;
@@ -462,7 +478,6 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     const float4 vi = vel[i];
     const float p_i = pw[i].x;
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
-    const int iy = cell_axis(pi.y, P.gmin[1], P.cell, P.dims[1]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float m_spiky_half = P.m * P.spiky * 0.5f;
     const float m_lap = P.m * P.lap;
@@ -470,7 +485,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
-        column_range(P, offset, ix, iy, iz, c, b, e);
+        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
         const uint2 mk = nbmask[(size_t)c * n + i];
         unsigned long long m = ((unsigned long long)mk.y << 32) | mk.x;
         // two neighbours per iteration: both loads are issued before either

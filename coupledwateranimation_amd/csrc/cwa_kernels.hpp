@@ -10,7 +10,8 @@ namespace cwa {
 
 struct GridDesc {
     float gmin[3];
-    float cell;
+    float cell;    // x and z cell edge
+    float cell_y;  // y cell edge
     int dims[3];
 };
 

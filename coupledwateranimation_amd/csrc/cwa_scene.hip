@@ -234,6 +234,7 @@ void validate_params(const cwa_params& p) {
     auto bad = [](const char* m) { throw Error(CWA_EINVAL, m); };
     if (!(p.h > 0.0f) || !std::isfinite(p.h)) bad("h must be > 0");
     if (!(p.cell_size >= p.h)) bad("cell_size must be >= h (27-cell search)");
+    if (p.grid_ydiv < 0 || p.grid_ydiv > 16) bad("grid_ydiv must be in 0..16");
     for (int a = 0; a < 3; ++a)
         if (p.grid_dims[a] < 1) bad("grid_dims must be >= 1");
     const double nc = (double)p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2];
@@ -247,6 +248,9 @@ void validate_params(const cwa_params& p) {
 
 // Smallest float x with sqrtf(x) >= h, so that  sqrt_rn(r2) < h  <=>  r2 < x
 // (sqrt_rn is monotone).  Host sqrtf is IEEE correctly rounded.
+// grid_ydiv with 0 read as 1 (zero-initialised parameter blocks).
+static int ydiv_of(const cwa_params& p) { return p.grid_ydiv <= 0 ? 1 : p.grid_ydiv; }
+
 static float exact_r2_threshold(float h) {
     float x = h * h;
     while (x > 0.0f && std::sqrt(x) >= h) x = std::nextafter(x, 0.0f);
@@ -311,6 +315,16 @@ DevParams make_devparams(const cwa_params& p, int W, int H) {
     d.kappa = p.couple_strength;
     d.m_rho0 = p.mass / p.rest_rho;
     d.cell = p.cell_size;
+    d.cell_y = p.cell_size / (float)ydiv_of(p);
+    {   // margin of the trimmed candidate rows: far above the rounding of
+        // the float bounds at the grid's coordinate magnitudes
+        float ext = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            const float edge = a == 1 ? d.cell_y : d.cell;
+            ext = std::max(ext, std::fabs(p.grid_min[a]) + (float)p.grid_dims[a] * edge);
+        }
+        d.grid_eps = 4e-6f * ext;
+    }
     d.inv_area = 1.0f / (p.cell_size * p.cell_size);
     // texel -> column-grid coordinates (computed in double, rounded once)
     const double Wd = W > 0 ? W : 1, Hd = H > 0 ? H : 1, cell = p.cell_size;
@@ -326,6 +340,7 @@ DevParams make_devparams(const cwa_params& p, int W, int H) {
 
 static GridDesc grid_desc(const cwa_params& p) {
     GridDesc g{};
+    g.cell_y = p.cell_size / (float)ydiv_of(p);
     for (int a = 0; a < 3; ++a) {
         g.gmin[a] = p.grid_min[a];
         g.dims[a] = p.grid_dims[a];
@@ -711,6 +726,7 @@ void cwa_params_reference(cwa_params* p) {
         p->grid_dims[a] =
             std::max(1, (int)std::ceil((p->upper[a] - p->lower[a]) / p->cell_size - 1e-4f));
     }
+    p->grid_ydiv = 1;
     p->check_finite = 0;
 }
 
@@ -919,6 +935,7 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
             g.dims[a] = dims[a];
         }
         g.cell = cell_size;
+        g.cell_y = cell_size;
         const int nc = dims[0] * dims[1] * dims[2];
         DeviceBuffer<int> trank((size_t)std::max(n, 1)), tmp((size_t)std::max(n, 1)),
             ws(scan_workspace_ints(nc));

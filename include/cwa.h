@@ -108,10 +108,15 @@ typedef struct cwa_params {
     float couple_band;
     float couple_strength;
     /* --- uniform grid (Ugrid3D, UniformGridParticles3D.h:60-67):
-     *     cell = clamp(floor((p - grid_min) / cell_size), 0, dims - 1) --- */
+     *     cell = clamp(floor((p - grid_min) / edge), 0, dims - 1), with
+     *     edge = cell_size along x and z and cell_size / grid_ydiv along y --- */
     float grid_min[3];
-    float cell_size;        /* must be >= h for the 27-cell search to be exact */
+    float cell_size;        /* x and z cell edge; must be >= h */
     int32_t grid_dims[3];   /* cells along x, y, z */
+    int32_t grid_ydiv;      /* y rows per cell_size (1..16; 0 means 1).  Finer
+                               rows let the neighbour search keep, in each of
+                               the 9 columns, only the rows the support sphere
+                               reaches */
     int32_t check_finite;   /* nonzero: step reports CWA_ENONFINITE on NaN/Inf */
 } cwa_params;
 

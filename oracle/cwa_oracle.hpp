@@ -107,7 +107,8 @@ struct Consts {
     R org[2], uvs[2];
     R lx, lz, atten, beta, type, wake_thr, wake_amt;
     R band, kappa;
-    R gmin[3], cell;
+    R gmin[3], cell, cell_y;
+    int ydiv;
     int dims[3];
     // column grid -> wave texel affine maps (see wave_coupling_coords)
     R ax, bx, az, bz;
@@ -166,6 +167,8 @@ static Consts<R> make_consts(const cwa_params& P, int wave_w, int wave_h) {
     c.band = R(P.couple_band);
     c.kappa = R(P.couple_strength);
     c.cell = R(P.cell_size);
+    c.ydiv = P.grid_ydiv <= 0 ? 1 : P.grid_ydiv;
+    c.cell_y = R(P.cell_size / (float)c.ydiv);  // the float edge the kernels use
     // Texel i of the wave has world x = origin + (i + 1/2)/(W * scale); its
     // column-grid coordinate is (x - grid_min)/cell - 1/2.  Computed in
     // double and rounded once.
@@ -198,7 +201,7 @@ static inline void cell_of(const Consts<R>& c, R px, R py, R pz, int& ix, int& i
     const R p[3] = {px, py, pz};
     int ci[3];
     for (int a = 0; a < 3; ++a) {
-        R f = std::floor((p[a] - c.gmin[a]) / c.cell);
+        R f = std::floor((p[a] - c.gmin[a]) / (a == 1 ? c.cell_y : c.cell));
         // clamp before converting so far-out points stay defined
         f = std::min(std::max(f, R(-1)), R(c.dims[a]));
         int q = (int)f;
@@ -285,11 +288,12 @@ static void grid_build(const Consts<R>& c, const R* pos4, int n, int32_t* key, i
 }
 
 // Visit the candidates of particle slot i in sorted order: the 9 columns
-// (dx outer, dz inner), each the contiguous cell range iy-1..iy+1.
+// (dx outer, dz inner), each the contiguous row range iy-ydiv..iy+ydiv
+// (one cell_size above and below; cell_size >= h).
 template <class R, class F>
 static inline void for_each_candidate(const Consts<R>& c, const int32_t* offset, int ix, int iy,
                                       int iz, F&& f) {
-    const int ylo = std::max(iy - 1, 0), yhi = std::min(iy + 1, c.dims[1] - 1);
+    const int ylo = std::max(iy - c.ydiv, 0), yhi = std::min(iy + c.ydiv, c.dims[1] - 1);
     for (int dx = -1; dx <= 1; ++dx) {
         const int cx = ix + dx;
         if (cx < 0 || cx >= c.dims[0]) continue;

@@ -129,7 +129,7 @@ def test_workload_builders(cwa):
     assert s.params.lambda_x == pytest.approx((1e-3 * 128) ** 2, rel=1e-6)
     big = configs.dam_break_large(steps=1)
     assert big.n == 1 << 20 and big.wave_size == (2048, 2048)
-    assert list(big.params.grid_dims) == [113, 57, 57]
+    assert list(big.params.grid_dims) == [113, 113, 57] and big.params.grid_ydiv == 2
     sp = big.info["spacing"]
     assert big.pos[:, 0].max() < 0.82 and big.pos[:, 1].max() < 1.0
     assert (4 / 3) * np.pi * (big.params.h / sp) ** 3 == pytest.approx(31, rel=0.02)

@@ -68,8 +68,8 @@ def _rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-@pytest.mark.parametrize("couple", [True, False])
-def test_few_step_adjoint_all_seeds(gpu, cwa, oracle, couple):
+@pytest.mark.parametrize("couple,ydiv", [(True, 1), (False, 1), (True, 4)])
+def test_few_step_adjoint_all_seeds(gpu, cwa, oracle, couple, ydiv):
     """Loss = <a, h^n> + <b, h^(n-1)> + <c, x> + <d, v> after 3 steps of a
     256-particle scene: parameter gradients and directional derivatives in
     the initial positions and velocities against FP64 FD."""
@@ -77,6 +77,8 @@ def test_few_step_adjoint_all_seeds(gpu, cwa, oracle, couple):
     from coupledwateranimation_amd.gradient import Tape
 
     spec = configs.dam_break_diff(n=256, wave=16, dt=2e-3, steps=3)
+    if ydiv != 1:
+        configs.grid_for_box(spec.params, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), spec.params.h, ydiv)
     if not couple:
         spec.params.couple_band = 0.0
     rng = np.random.default_rng(5)

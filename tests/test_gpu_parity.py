@@ -115,7 +115,7 @@ def test_grid_build_bit_exact_1m(gpu, cwa, oracle):
     import torch
     from coupledwateranimation_amd import configs
 
-    s = configs.dam_break_large(steps=1)
+    s = configs.dam_break_large(steps=1, ydiv=1)  # cwa_grid_build: cubic cells
     _say("grid 1M")
     g = _gpu_grid(cwa, torch, gpu, s.params, s.pos)
     r = oracle.grid_build(s.params, s.pos)
@@ -199,10 +199,13 @@ def _oracle_run(oracle, spec, steps, double=False, threads=8):
 def _specs(cwa):
     from coupledwateranimation_amd import configs
 
-    return {"dam_break_4k": configs.dam_break_small(), "reference": configs.reference_scene()}
+    fine = configs.dam_break_small()
+    configs.grid_for_box(fine.params, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), fine.params.h, 4)
+    return {"dam_break_4k": configs.dam_break_small(), "reference": configs.reference_scene(),
+            "dam_break_4k_ydiv4": fine}
 
 
-@pytest.mark.parametrize("name", ["dam_break_4k", "reference"])
+@pytest.mark.parametrize("name", ["dam_break_4k", "reference", "dam_break_4k_ydiv4"])
 def test_one_step_parity(gpu, cwa, oracle, name):
     """One coupled step: grid arrays bit-exact; rho relative 2e-6; pressure
     p = k (rho - rho0) + H rho within 2e-6 (k max rho + max |p|) (the
@@ -227,11 +230,13 @@ def test_one_step_parity(gpu, cwa, oracle, name):
                        atol=1e-6 * max(np.abs(og["colflux"]).max(), 1e-30))
 
 
-def test_n_step_parity_vs_fp64(gpu, cwa, oracle):
-    """50 coupled steps of the 4 096-particle dam break.  Error of the GPU
-    against the FP64 oracle, per field, must stay within 20x the FP32
-    oracle's own error plus a floor of 1e-6 (relative to the field max)."""
-    spec = _specs(cwa)["dam_break_4k"]
+@pytest.mark.parametrize("name", ["dam_break_4k", "dam_break_4k_ydiv4"])
+def test_n_step_parity_vs_fp64(gpu, cwa, oracle, name):
+    """50 coupled steps of the 4 096-particle dam break (y rows of h and of
+    h/4).  Error of the GPU against the FP64 oracle, per field, must stay
+    within 20x the FP32 oracle's own error plus a floor of 1e-6 (relative
+    to the field max)."""
+    spec = _specs(cwa)[name]
     steps = 50
     st, _ = _gpu_run(cwa, spec, steps)
     _, f32 = _oracle_run(oracle, spec, steps)

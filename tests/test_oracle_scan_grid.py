@@ -104,15 +104,17 @@ def test_cell_hash_edge_cases(oracle, cwa):
                                  lin(3, 2, 1), lin(1, 0, 1)]
 
 
+@pytest.mark.parametrize("ydiv", [1, 3, 4])
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_neighbor_sets_grid_equals_all_pairs(oracle, cwa, seed):
+def test_neighbor_sets_grid_equals_all_pairs(oracle, cwa, seed, ydiv):
     """The 27-cell search returns exactly the all-pairs set r < h
-    (rho_pres_comp.glsl:60-64), including cells of size > h and points
-    outside the grid (clamped into edge cells)."""
+    (rho_pres_comp.glsl:60-64), including cells of size > h, y rows of
+    cell/ydiv, and points outside the grid (clamped into edge cells)."""
     rng = np.random.default_rng(seed)
     n = 3000
     h = 0.05
-    p = _params(cwa, dims=(8, 9, 7), cell=0.06 + 0.01 * seed, h=h)
+    p = _params(cwa, dims=(8, 9 * ydiv, 7), cell=0.06 + 0.01 * seed, h=h)
+    p.grid_ydiv = ydiv
     pos = np.ones((n, 4), np.float32)
     pos[:, :3] = rng.uniform(-0.05, 0.55, (n, 3)).astype(np.float32)
     # near-duplicates at distance ~h to exercise the strict comparison
