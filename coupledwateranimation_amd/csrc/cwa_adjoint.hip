@@ -99,32 +99,22 @@ __device__ __forceinline__ bool coupled_(const DevParams& P, float y, float H) {
     return !(P.band > 0.0f) || fabsf(y - H) < P.band;
 }
 
-// Visit the frozen neighbours of particle i (bit order), excluding i.
+// Visit the frozen neighbours of particle i (the density pass's neighbour
+// words, candidate order), excluding i.  st: the thread's LDS staging slots
+// (kNbStage of them, stride 256).
 template <class F>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
                                                    const int* __restrict__ offset,
-                                                   const uint2* __restrict__ nbmask,
+                                                   const uint2* __restrict__ nbw,
                                                    const float4* __restrict__ pos, int n, int i,
-                                                   float4 pi, F&& f) {
-    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
-    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
-#pragma unroll 1
-    for (int c = 0; c < 9; ++c) {
-        int b, e;
-        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
-        const uint2 mk = nbmask[(size_t)c * n + i];
-        unsigned long long m = ((unsigned long long)mk.y << 32) | mk.x;
-        while (m) {
-            const int j = b + __builtin_ctzll(m);
-            m &= m - 1ull;
-            if (j != i) f(j);
-        }
-        for (int j = b + 64; j < e; ++j) {
-            const float4 pj = pos[j];
-            const float r2 = dist2_fma(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            if (r2 < P.r2_lt && j != i) f(j);
-        }
+                                                   float4 pi, uint2* st, F&& f) {
+    const int nw = stage_nb_words(nbw, n, i, st, 256);
+    if (nw < 0) {
+        for_each_candidate_neighbour(P, offset, pos, i, pi, f);
+        return;
     }
+    NbCursor nb(st, 256, nw);
+    for (int j = nb.next(); j >= 0; j = nb.next()) f(j);
 }
 
 }  // namespace
@@ -374,13 +364,14 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
 __global__ void __launch_bounds__(256) k_adj_pairs(
     const DevParams* __restrict__ Pd, float dt, int n, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const int* __restrict__ offset,
-    const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
+    const uint2* __restrict__ nbw, const float4* __restrict__ nrec,
     const float* __restrict__ rho_w, const float2* __restrict__ pw,
     const float* __restrict__ rho_raw, const float* __restrict__ H0,
     const float* __restrict__ Sbar, const float4* __restrict__ adjrec,
     const float* __restrict__ rhob_d, float* __restrict__ G0n, float4* __restrict__ xb_loc,
     float4* __restrict__ vb_loc, float* __restrict__ rtb, double* __restrict__ acc) {
     const DevParams& P = *Pd;
+    __shared__ uint2 s_nb[kNbStage][256];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     float part[3] = {0.f, 0.f, 0.f};  // visc, gas_const, rest_rho
     if (k < n) {
@@ -392,7 +383,7 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
         float pkb = 0.f, irkb = 0.f;
         float xbx = 0.f, xby = 0.f, xbz = 0.f, vbx = 0.f, vby = 0.f, vbz = 0.f;
         float Vx = 0.f, Vy = 0.f, Vz = 0.f;  // viscosity sum of k (for d/d visc)
-        for_each_neighbour(P, offset, nbmask, pos, n, k, xk, [&](int j) {
+        for_each_neighbour(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], [&](int j) {
             const float4 ra = nrec[2 * (size_t)j], rb = nrec[2 * (size_t)j + 1];
             const float pj = ra.w, irj = rb.w;
             const float4 Fbj = adjrec[2 * (size_t)j], Vbj = adjrec[2 * (size_t)j + 1];
@@ -508,17 +499,18 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_rho_pairs(
     const DevParams* __restrict__ Pd, int n, const float4* __restrict__ pos, const int* __restrict__ offset,
-    const uint2* __restrict__ nbmask, const float* __restrict__ rtb,
+    const uint2* __restrict__ nbw, const float* __restrict__ rtb,
     const float4* __restrict__ xb_loc, const float4* __restrict__ vb_loc,
     const int* __restrict__ list, float4* __restrict__ xb_out, float4* __restrict__ vb_out) {
     const DevParams& P = *Pd;
+    __shared__ uint2 s_nb[kNbStage][256];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 xk = pos[k];
     const float bk = rtb[k];
     const float c = P.m_poly6 * 3.0f * -2.0f;
     float4 xb = xb_loc[k];
-    for_each_neighbour(P, offset, nbmask, pos, n, k, xk, [&](int j) {
+    for_each_neighbour(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], [&](int j) {
         const float4 pj = pos[j];
         const float dx = xk.x - pj.x, dy = xk.y - pj.y, dz = xk.z - pj.z;
         const float r2 = dist2_fma(dx, dy, dz);
@@ -616,25 +608,25 @@ hipError_t launch_adj_local(const DevParams& P, float dt, int n, const float4* p
 }
 
 hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* pos,
-                            const float4* vel, const int* offset, const uint2* nbmask,
+                            const float4* vel, const int* offset, const uint2* nbw,
                             const float4* nrec, const float* rho_w, const float2* pw,
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, dt, n, pos, vel, offset, nbmask, nrec, rho_w,
+    k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w,
                                               pw, rho_raw, H0, Sbar, adjrec, rhob_d, G0n,
                                               xb_loc, vb_loc, rtb, acc);
     return hipGetLastError();
 }
 
 hipError_t launch_adj_rho_pairs(const DevParams& P, int n, const float4* pos, const int* offset,
-                                const uint2* nbmask, const float* rtb, const float4* xb_loc,
+                                const uint2* nbw, const float* rtb, const float4* xb_loc,
                                 const float4* vb_loc, const int* list, float4* xb_out,
                                 float4* vb_out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_rho_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, n, pos, offset, nbmask, rtb, xb_loc, vb_loc,
+    k_adj_rho_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, n, pos, offset, nbw, rtb, xb_loc, vb_loc,
                                                   list, xb_out, vb_out);
     return hipGetLastError();
 }

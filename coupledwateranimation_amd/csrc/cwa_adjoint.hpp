@@ -40,14 +40,14 @@ hipError_t launch_adj_local(const DevParams& P, float dt, int n, const float4* p
                             float* G0n, float4* adjrec, float* rhob_d, float4* xb_loc,
                             float4* vb_loc, hipStream_t st);
 hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* pos,
-                            const float4* vel, const int* offset, const uint2* nbmask,
+                            const float4* vel, const int* offset, const uint2* nbw,
                             const float4* nrec, const float* rho_w, const float2* pw,
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st);
 hipError_t launch_adj_rho_pairs(const DevParams& P, int n, const float4* pos, const int* offset,
-                                const uint2* nbmask, const float* rtb, const float4* xb_loc,
+                                const uint2* nbw, const float* rtb, const float4* xb_loc,
                                 const float4* vb_loc, const int* list, float4* xb_out,
                                 float4* vb_out, hipStream_t st);
 

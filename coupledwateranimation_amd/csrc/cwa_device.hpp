@@ -109,6 +109,99 @@ __device__ __forceinline__ void column_range(const DevParams& P, const int* __re
 }
 
 // ---------------------------------------------------------------------------
+// Neighbour words: the exact neighbour set of a particle as recorded by the
+// density pass, for the passes that visit it again (force, adjoint).
+// Per candidate column c (0..8) of particle i:
+//   nbw[c*n + i]     = (bits of candidates 0..31, b | flags)
+//   nbw[(9+c)*n + i] = (bits of candidates 32..63, b + 32)   only if kNbHi
+// with b the column's first candidate slot and bit k set when slot base + k
+// is a neighbour other than i itself.  kNbHi marks a column with a
+// neighbour among candidates 32..63; kNbTail (on column 0's entry) marks a
+// particle with a column longer than 64 candidates, whose neighbours the
+// consumers re-search with the exact distance test instead.
+// ---------------------------------------------------------------------------
+constexpr int kNbEntries = 18;
+constexpr uint32_t kNbHi = 1u << 31;
+constexpr uint32_t kNbTail = 1u << 30;
+constexpr uint32_t kNbSlot = kNbTail - 1u;
+
+// Up to kNbStage non-zero words of one particle, staged per thread in LDS
+// (slot t of the thread at s[t * stride]) by stage_nb_words, then walked in
+// order by NbCursor: one flat loop over all columns, so a wave runs as many
+// iterations as its busiest lane has neighbours rather than the sum over
+// columns of each column's busiest lane.
+constexpr int kNbStage = 12;
+
+// Number of words staged, or -1 when the particle must be re-searched
+// (a column past 64 candidates, or more than kNbStage non-zero words).
+__device__ __forceinline__ int stage_nb_words(const uint2* __restrict__ nbw, int n, int i,
+                                              uint2* s, int stride) {
+    uint2 e[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) e[c] = nbw[(size_t)c * n + i];
+    if (e[0].y & kNbTail) return -1;
+    int nw = 0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const uint32_t b = e[c].y & kNbSlot;
+        if (e[c].x) {
+            if (nw < kNbStage) s[nw * stride] = make_uint2(e[c].x, b);
+            ++nw;
+        }
+        if (e[c].y & kNbHi) {
+            if (nw < kNbStage) s[nw * stride] = nbw[(size_t)(9 + c) * n + i];
+            ++nw;
+        }
+    }
+    return nw > kNbStage ? -1 : nw;
+}
+
+struct NbCursor {
+    const uint2* s;
+    int stride, nw, t;
+    uint32_t cur;  // bits left in the current word
+    int base;
+
+    __device__ __forceinline__ NbCursor(const uint2* st, int stride_, int nw_)
+        : s(st), stride(stride_), nw(nw_), t(0), cur(0u), base(0) {}
+    // Slot of the next neighbour, -1 when none is left.
+    __device__ __forceinline__ int next() {
+        if (cur == 0u && t < nw) {
+            const uint2 w = s[t * stride];
+            cur = w.x;
+            base = (int)w.y;
+            ++t;
+        }
+        if (cur == 0u) return -1;
+        const int j = base + __builtin_ctz(cur);
+        cur &= cur - 1u;
+        return j;
+    }
+};
+
+// Neighbours of particle i (slot i, position pi) in candidate order by the
+// exact distance test, excluding i: the path for particles whose words
+// overflowed.
+template <class F>
+__device__ __forceinline__ void for_each_candidate_neighbour(const DevParams& P,
+                                                             const int* __restrict__ offset,
+                                                             const float4* __restrict__ pos,
+                                                             int i, float4 pi, F&& f) {
+    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
+    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+#pragma unroll 1
+    for (int c = 0; c < 9; ++c) {
+        int b, e;
+        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
+        for (int j = b; j < e; ++j) {
+            const float4 pj = pos[j];
+            const float r2 = dist2_fma(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            if (r2 < P.r2_lt && j != i) f(j);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // GL_LINEAR / CLAMP_TO_EDGE sampling of a single-channel fp32 image.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float fetch_clamped(const float* __restrict__ img, int w, int h, int i,

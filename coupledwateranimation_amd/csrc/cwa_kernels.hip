@@ -345,9 +345,9 @@ __device__ __forceinline__ void density_candidate(const float4 pi, const float4 
 
 // ---------------------------------------------------------------------------
 // Density / pressure (rho_pres_comp.glsl:49-81) + particle flux term.
-// Also records the exact neighbour set of every particle: per candidate
-// column a 64-bit mask over its first 64 candidates (nbmask[c*n + i];
-// candidates past 64 are re-tested by the force pass).
+// Also records the exact neighbour set of every particle as neighbour words
+// (nbw: per candidate column the bits of its first 64 candidates, the
+// particle itself left out; layout and flags at kNbEntries).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
                                                  float* __restrict__ rho_w,
                                                  float2* __restrict__ pw,
                                                  float* __restrict__ flux,
-                                                 uint2* __restrict__ nbmask,
+                                                 uint2* __restrict__ nbw,
                                                  float4* __restrict__ nrec,
                                                  float* __restrict__ rho_raw) {
     const DevParams& P = *Pd;
@@ -368,6 +368,8 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
+    uint2 e0 = make_uint2(0u, 0u);  // column 0's entry, written last
+    bool tail = false;
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
@@ -388,9 +390,17 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
             for (k = 32; k < l2; ++k) density_candidate(pi, q[k], T, acc, hi, k - 32);
             uint32_t none = 0;
             for (k = 64; k < len; ++k) density_candidate(pi, q[k], T, acc, none, 0);
+            tail |= len > 64;
         }
-        nbmask[(size_t)c * n + i] = make_uint2(lo, hi);
+        const uint32_t self = (uint32_t)(i - b);  // i is a candidate of one column
+        if (self < 32u) lo &= ~(1u << self);
+        else if (self < 64u) hi &= ~(1u << (self - 32u));
+        const uint2 ent = make_uint2(lo, (uint32_t)b | (hi ? kNbHi : 0u));
+        if (c == 0) e0 = ent;
+        else nbw[(size_t)c * n + i] = ent;
+        if (hi) nbw[(size_t)(9 + c) * n + i] = make_uint2(hi, (uint32_t)b + 32u);
     }
+    nbw[i] = make_uint2(e0.x, e0.y | (tail ? kNbTail : 0u));
     float rho = P.m_poly6 * acc;
     if (rho_raw) rho_raw[i] = rho;  // taped for the adjoint
     float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
@@ -437,7 +447,7 @@ __device__ __forceinline__ Nbr load_nbr(const float4* __restrict__ nrec, int j) 
     return Nbr{nrec[2 * (size_t)j], nrec[2 * (size_t)j + 1]};
 }
 
-__device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4 vi, float p_i,
+__device__ __forceinline__ void force_pair(float h, float4 pi, float4 vi, float p_i,
                                            float m_spiky_half, float m_lap, const Nbr& nb,
                                            PairAcc& a) {
     const float dx = pi.x - nb.a.x, dy = pi.y - nb.a.y, dz = pi.z - nb.a.z;
@@ -446,7 +456,7 @@ __device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4
     // pressure direction: rinv = 0 drops that term
     const float rinv = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float r = r2 * rinv;
-    const float hr = P.h - r;
+    const float hr = h - r;
     const float s = m_spiky_half * (p_i + nb.a.w) * nb.b.w * hr * hr * rinv;
     a.pfx -= s * dx;
     a.pfy -= s * dy;
@@ -460,14 +470,14 @@ __device__ __forceinline__ void force_pair(const DevParams& P, float4 pi, float4
 // ---------------------------------------------------------------------------
 // Forces (force_comp.glsl:62-139) fused with symplectic Euler + boundary
 // (integrate_comp.glsl:56-92, 135-178).  Visits only the neighbours the
-// density pass recorded (bit order = candidate order), reads the sorted
-// copy B and writes the state A in the same slot order.
+// density pass recorded (its neighbour words, in candidate order), reads
+// the sorted copy B and writes the state A in the same slot order.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_force_integrate(
     const DevParams* __restrict__ Pd, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
     int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
-    const uint2* __restrict__ nbmask, const float4* __restrict__ nrec,
+    const uint2* __restrict__ nbw, const float4* __restrict__ nrec,
     const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
     float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre) {
@@ -477,36 +487,32 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     const float4 pi = pos[i];
     const float4 vi = vel[i];
     const float p_i = pw[i].x;
-    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
-    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+    const float h = P.h;
     const float m_spiky_half = P.m * P.spiky * 0.5f;
     const float m_lap = P.m * P.lap;
     PairAcc a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-    for (int c = 0; c < 9; ++c) {
-        int b, e;
-        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
-        const uint2 mk = nbmask[(size_t)c * n + i];
-        unsigned long long m = ((unsigned long long)mk.y << 32) | mk.x;
-        // two neighbours per iteration: both loads are issued before either
-        // pair is evaluated; pairs are still accumulated in bit order
-        while (m) {
-            const int j0 = b + __builtin_ctzll(m);
-            m &= m - 1ull;
-            const bool two = m != 0ull;
-            const int j1 = two ? b + __builtin_ctzll(m) : j0;
-            m &= two ? m - 1ull : m;
+    __shared__ uint2 s_nb[kNbStage][256];
+    uint2* st = &s_nb[0][threadIdx.x];
+    const int nw = stage_nb_words(nbw, n, i, st, 256);
+    if (nw >= 0) {
+        // two neighbours per iteration: both records are loaded before
+        // either pair is evaluated (the second pair is evaluated with zero
+        // weight when the list ran out); pairs are accumulated in order
+        NbCursor nb(st, 256, nw);
+        for (;;) {
+            const int j0 = nb.next();
+            if (j0 < 0) break;
+            const int j1 = nb.next();
             const Nbr n0 = load_nbr(nrec, j0);
-            const Nbr n1 = load_nbr(nrec, j1);
-            if (j0 != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, n0, a);
-            if (two && j1 != i) force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, n1, a);
+            const Nbr n1 = load_nbr(nrec, j1 < 0 ? j0 : j1);
+            force_pair(h, pi, vi, p_i, m_spiky_half, m_lap, n0, a);
+            force_pair(h, pi, vi, p_i, j1 < 0 ? 0.0f : m_spiky_half, j1 < 0 ? 0.0f : m_lap, n1,
+                       a);
         }
-        for (int j = b + 64; j < e; ++j) {  // columns longer than 64 candidates
-            const float4 pj = pos[j];
-            const float r2 = dist2_fma(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            if (r2 < P.r2_lt && j != i)
-                force_pair(P, pi, vi, p_i, m_spiky_half, m_lap, load_nbr(nrec, j), a);
-        }
+    } else {
+        for_each_candidate_neighbour(P, offset, pos, i, pi, [&](int j) {
+            force_pair(h, pi, vi, p_i, m_spiky_half, m_lap, load_nbr(nrec, j), a);
+        });
     }
     float vfx = a.vfx, vfy = a.vfy, vfz = a.vfz;
     float4 fp = force[i];
@@ -755,22 +761,22 @@ hipError_t launch_stable_reorder(const int* key, const int* offset, const int* t
 }
 
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
-                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
+                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbw,
                           float4* nrec, float* rho_raw, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, B.pos, B.vel, offset, n, H0, rho_w, pw,
-                                                  flux, nbmask, nrec, rho_raw);
+                                                  flux, nbw, nrec, rho_raw);
     return hipGetLastError();
 }
 
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const uint2* nbmask, const float4* nrec, const float* H0,
+                                  const uint2* nbw, const float4* nrec, const float* H0,
                                   ParticleArrays& A, float* rho, float* prs, float4* fpre,
                                   hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, dt, B.pos, B.vel, B.force, B.id,
-                                                          offset, n, rho_w, pw, nbmask, nrec, H0,
+                                                          offset, n, rho_w, pw, nbw, nrec, H0,
                                                           A.pos, A.vel, A.force, A.id, rho, prs,
                                                           fpre);
     return hipGetLastError();

@@ -36,16 +36,16 @@ hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offse
 hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
                                  int* list, const ParticleArrays* in, ParticleArrays* out,
                                  hipStream_t st);
-// nbmask (9*n uint2) and nrec (2*n float4) are written by the density pass
+// nbw (kNbEntries*n uint2, neighbour words) and nrec (2*n float4) are written by the density pass
 // and read by the force pass.  rho_raw (density before the wave term and the
 // rest-density clamp) and fpre (force before the foam branch) are optional
 // outputs for the adjoint tape (null: not written).
 hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
-                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbmask,
+                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbw,
                           float4* nrec, float* rho_raw, hipStream_t st);
 hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
                                   const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const uint2* nbmask, const float4* nrec, const float* H0,
+                                  const uint2* nbw, const float4* nrec, const float* H0,
                                   ParticleArrays& A, float* rho, float* prs, float4* fpre,
                                   hipStream_t st);
 hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],

@@ -207,12 +207,12 @@ class SphParticles3D {
     float* rho_w() const { return rho_w_.get(); }
     float2* pw() const { return pw_.get(); }
     float* flux() const { return flux_.get(); }
-    uint2* nbmask() const { return nbmask_.get(); }
+    uint2* nbw() const { return nbw_.get(); }
     float4* nrec() const { return nrec_.get(); }
 
    private:
     int n_ = 0;
-    DeviceBuffer<uint2> nbmask_;  // 9 x n neighbour bitmasks of the last step
+    DeviceBuffer<uint2> nbw_;  // kNbEntries x n neighbour words of the last step
     DeviceBuffer<float4> nrec_;   // 2 x n neighbour records of the last step
     DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
     DeviceBuffer<int> id_[2];
@@ -230,7 +230,7 @@ struct TapeOut {
     float4* vel = nullptr;
     int* list = nullptr;       // stable cell order of the step
     int* offset = nullptr;     // cell offsets of the step
-    uint2* nbmask = nullptr;   // neighbour masks
+    uint2* nbw = nullptr;      // neighbour words
     float4* nrec = nullptr;    // neighbour records
     float* rho = nullptr;      // density (after the clamp)
     float2* pw = nullptr;      // (pressure, 1/rho)

@@ -190,7 +190,7 @@ void SphParticles3D::Allocate(int n) {
     rho_w_.resize(m);
     flux_.resize(m);
     pw_.resize(m);
-    nbmask_.resize(9 * m);
+    nbw_.resize((size_t)kNbEntries * m);
     nrec_.resize(2 * m);
     A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
     B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
@@ -354,6 +354,7 @@ static GridDesc grid_desc(const cwa_params& p) {
 // ---------------------------------------------------------------------------
 static void validate_init(const cwa_init& in) {
     if (in.n < 0) throw Error(CWA_EINVAL, "n < 0");
+    if (in.n > CWA_MAX_PARTICLES) throw Error(CWA_EINVAL, "n > CWA_MAX_PARTICLES");
     if (in.n > 0 && !in.pos) throw Error(CWA_EINVAL, "pos is null");
     if (in.wave_w < 1 || in.wave_h < 1) throw Error(CWA_EINVAL, "wave size must be >= 1x1");
 }
@@ -444,7 +445,7 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
     B.vel = pick(tape ? tape->vel : nullptr, particles.B.vel);
     int* off = pick(tape ? tape->offset : nullptr, grid.offset());
     int* lst = pick(tape ? tape->list : nullptr, grid.list());
-    uint2* nbm = pick(tape ? tape->nbmask : nullptr, particles.nbmask());
+    uint2* nbm = pick(tape ? tape->nbw : nullptr, particles.nbw());
     float4* nrec = pick(tape ? tape->nrec : nullptr, particles.nrec());
     float* rho_w = pick(tape ? tape->rho : nullptr, particles.rho_w());
     float2* pw = pick(tape ? tape->pw : nullptr, particles.pw());

@@ -48,7 +48,7 @@ class Tape {
     struct StepTape {
         DeviceBuffer<float4> pos, vel, nrec, fpre;
         DeviceBuffer<int> list, offset;
-        DeviceBuffer<uint2> nbmask;
+        DeviceBuffer<uint2> nbw;
         DeviceBuffer<float> rho, rho_raw, h0, h1, colflux;
         DeviceBuffer<float2> pw;
     };
@@ -109,7 +109,7 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
         t.fpre.resize(n);
         t.list.resize(n);
         t.offset.resize((size_t)ncell_ + 1);
-        t.nbmask.resize(9 * n);
+        t.nbw.resize((size_t)kNbEntries * n);
         t.rho.resize(n);
         t.rho_raw.resize(n);
         t.pw.resize(n);
@@ -236,7 +236,7 @@ void Tape::record_step(StepTape& t) {
     out.vel = t.vel.get();
     out.list = t.list.get();
     out.offset = t.offset.get();
-    out.nbmask = t.nbmask.get();
+    out.nbw = t.nbw.get();
     out.nrec = t.nrec.get();
     out.rho = t.rho.get();
     out.pw = t.pw.get();
@@ -258,12 +258,12 @@ void Tape::adjoint_step(const StepTape& t) {
                                    adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st_),
                   "adj local");
         hip_check(launch_adj_pairs(P, dt_, n_, t.pos.get(), t.vel.get(), t.offset.get(),
-                                   t.nbmask.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
+                                   t.nbw.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
                                    t.rho_raw.get(), t.h0.get(), Sbar_.get(), adjrec_.get(),
                                    rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
                                    acc_pair_.get(), st_),
                   "adj pairs");
-        hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbmask.get(),
+        hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbw.get(),
                                        rtb_.get(), xl_.get(), vl_.get(), t.list.get(),
                                        xb2_.get(), vb2_.get(), st_),
                   "adj rho pairs");

@@ -50,6 +50,9 @@ extern "C" {
 #define CWA_MODE_INIT_FROM_TEXTURE 1
 #define CWA_MODE_EVOLVE 2
 
+/* Largest particle count of one scene (slot indices keep two spare bits). */
+#define CWA_MAX_PARTICLES (1 << 29)
+
 /*
  * Every constant of the reference shaders and uniform blocks, as a field.
  *   ConstantsUniform  Main.cpp:184-190   (mass, smoothing, visc, rest rho)
@@ -122,7 +125,7 @@ typedef struct cwa_params {
 
 /* Initial state handed to create / reinit (host memory). */
 typedef struct cwa_init {
-    int32_t n;              /* particle count (may be 0) */
+    int32_t n;              /* particle count, 0..CWA_MAX_PARTICLES */
     const float* pos;       /* 4*n floats, xyz + pad (Particle.pos, Main.cpp:169) */
     const float* vel;       /* 4*n floats or NULL for zero (Particle.vel) */
     const float* force;     /* 4*n floats or NULL for zero (Particle.force) */
