@@ -155,3 +155,22 @@ def test_invalid_init_rejected(cwa):
     p = cwa.reference_params()
     with pytest.raises(cwa.CwaError):
         cwa.Scene(p, np.zeros((4, 4), np.float32), wave_size=(0, 8))
+
+
+def test_particle_limit_rejected(cwa):
+    """n above CWA_MAX_PARTICLES fails with EINVAL before any device work
+    (slot indices of the neighbour records keep two flag bits)."""
+    import ctypes as C
+
+    from coupledwateranimation_amd import _lib
+
+    p = cwa.reference_params()
+    one = np.zeros(4, np.float32)
+    init = _lib.Init()
+    init.n = (1 << 29) + 1
+    init.pos = one.ctypes.data_as(C.POINTER(C.c_float))
+    init.wave_w = init.wave_h = 8
+    h = C.c_void_p()
+    rc = _lib.lib().cwa_scene_create(C.byref(p), C.byref(init), C.byref(h))
+    assert rc == _lib.CWA_EINVAL and not h.value
+    assert "CWA_MAX_PARTICLES" in _lib.lib().cwa_last_error().decode()
