@@ -100,22 +100,43 @@ __device__ __forceinline__ bool coupled_(const DevParams& P, float y, float H) {
 }
 
 // Visit the frozen neighbours of particle i (the density pass's neighbour
-// words, candidate order), excluding i.  st: the thread's LDS staging slots
-// (kNbStage of them, stride 256).
-template <class F>
+// words, candidate order), excluding i: load(j) fetches what one pair needs
+// and use(record) consumes it.  U records are loaded before the first of
+// them is used; records are used in order.  st: the thread's LDS staging
+// slots (kNbStage of them, stride 256).
+template <int U, class Load, class Use>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
                                                    const int* __restrict__ offset,
                                                    const uint2* __restrict__ nbw,
                                                    const float4* __restrict__ pos, int n, int i,
-                                                   float4 pi, uint2* st, F&& f) {
+                                                   float4 pi, uint2* st, Load&& load, Use&& use) {
     const int nw = stage_nb_words(nbw, n, i, st, 256);
     if (nw < 0) {
-        for_each_candidate_neighbour(P, offset, pos, i, pi, f);
+        for_each_candidate_neighbour(P, offset, pos, i, pi, [&](int j) { use(load(j)); });
         return;
     }
     NbCursor nb(st, 256, nw);
-    for (int j = nb.next(); j >= 0; j = nb.next()) f(j);
+    for (;;) {
+        int j[U];
+        j[0] = nb.next();
+        if (j[0] < 0) break;
+#pragma unroll
+        for (int u = 1; u < U; ++u) j[u] = nb.next();
+        decltype(load(0)) r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) r[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) use(r[u]);
+    }
 }
+
+// What the pair adjoint reads of neighbour j.
+struct AdjNbr {
+    float4 ra, rb;    // neighbour record (x, y, z, p), (vx, vy, vz, 1/rho)
+    float4 Fbj, Vbj;  // j's adjoint record
+};
 
 }  // namespace
 
@@ -383,10 +404,14 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
         float pkb = 0.f, irkb = 0.f;
         float xbx = 0.f, xby = 0.f, xbz = 0.f, vbx = 0.f, vby = 0.f, vbz = 0.f;
         float Vx = 0.f, Vy = 0.f, Vz = 0.f;  // viscosity sum of k (for d/d visc)
-        for_each_neighbour(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], [&](int j) {
-            const float4 ra = nrec[2 * (size_t)j], rb = nrec[2 * (size_t)j + 1];
+        const auto load = [&](int j) {
+            return AdjNbr{nrec[2 * (size_t)j], nrec[2 * (size_t)j + 1], adjrec[2 * (size_t)j],
+                          adjrec[2 * (size_t)j + 1]};
+        };
+        for_each_neighbour<2>(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], load,
+                              [&](const AdjNbr& q) {
+            const float4 ra = q.ra, rb = q.rb, Fbj = q.Fbj, Vbj = q.Vbj;
             const float pj = ra.w, irj = rb.w;
-            const float4 Fbj = adjrec[2 * (size_t)j], Vbj = adjrec[2 * (size_t)j + 1];
             const float dx = xk.x - ra.x, dy = xk.y - ra.y, dz = xk.z - ra.z;
             const float r2 = dist2_fma(dx, dy, dz);
             if (!(r2 > 0.0f)) return;  // coincident pair: no term in the forward
@@ -510,12 +535,13 @@ __global__ void __launch_bounds__(256) k_adj_rho_pairs(
     const float bk = rtb[k];
     const float c = P.m_poly6 * 3.0f * -2.0f;
     float4 xb = xb_loc[k];
-    for_each_neighbour(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], [&](int j) {
-        const float4 pj = pos[j];
+    const auto load = [&](int j) { return make_float4(pos[j].x, pos[j].y, pos[j].z, rtb[j]); };
+    for_each_neighbour<4>(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], load,
+                          [&](const float4& pj) {
         const float dx = xk.x - pj.x, dy = xk.y - pj.y, dz = xk.z - pj.z;
         const float r2 = dist2_fma(dx, dy, dz);
         const float w = fmaxf(P.r2_lt - r2, 0.0f);
-        const float s = c * w * w * (bk + rtb[j]);
+        const float s = c * w * w * (bk + pj.w);
         xb.x += s * dx;
         xb.y += s * dy;
         xb.z += s * dz;

@@ -495,19 +495,32 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     uint2* st = &s_nb[0][threadIdx.x];
     const int nw = stage_nb_words(nbw, n, i, st, 256);
     if (nw >= 0) {
-        // two neighbours per iteration: both records are loaded before
-        // either pair is evaluated (the second pair is evaluated with zero
-        // weight when the list ran out); pairs are accumulated in order
+        // kForceBatch neighbours per iteration: every record is loaded
+        // before any pair is evaluated (slots past the end of the list hold
+        // a zero record and are evaluated with zero weight); pairs are
+        // accumulated in order.  Four in flight measured fastest (MI355X,
+        // 1M-particle dam break; 1..8 tried).
+        constexpr int kForceBatch = 4;
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
         NbCursor nb(st, 256, nw);
         for (;;) {
-            const int j0 = nb.next();
-            if (j0 < 0) break;
-            const int j1 = nb.next();
-            const Nbr n0 = load_nbr(nrec, j0);
-            const Nbr n1 = load_nbr(nrec, j1 < 0 ? j0 : j1);
-            force_pair(h, pi, vi, p_i, m_spiky_half, m_lap, n0, a);
-            force_pair(h, pi, vi, p_i, j1 < 0 ? 0.0f : m_spiky_half, j1 < 0 ? 0.0f : m_lap, n1,
-                       a);
+            int j[kForceBatch];
+            j[0] = nb.next();
+            if (j[0] < 0) break;
+#pragma unroll
+            for (int u = 1; u < kForceBatch; ++u) j[u] = nb.next();
+            Nbr r[kForceBatch];
+#pragma unroll
+            for (int u = 0; u < kForceBatch; ++u) {
+                r[u] = Nbr{zero, zero};
+                if (j[u] >= 0) r[u] = load_nbr(nrec, j[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kForceBatch; ++u) {
+                const bool live = j[u] >= 0;
+                force_pair(h, pi, vi, p_i, live ? m_spiky_half : 0.0f, live ? m_lap : 0.0f, r[u],
+                           a);
+            }
         }
     } else {
         for_each_candidate_neighbour(P, offset, pos, i, pi, [&](int j) {
