@@ -2,7 +2,7 @@
 //
 // The forward step is differentiated as the code runs it, with every
 // discrete decision frozen at its forward value: the cell sort and the
-// neighbour sets (bitmasks of the forward density pass), the branches of
+// neighbour sets (neighbour words of the forward density pass), the branches of
 // max(k(rho - rho0), 0) and max(rho0, rho), the coupling band gate, foam,
 // surface clamp, box clamps and the crest factor.  The reference's torque
 // term is not differentiated (the adjoint requires torque_coeff == 0).

@@ -5,7 +5,7 @@
 // SoA + id + the two newest heightfields) and once more at the end.
 // Backward: for each segment, last to first, the segment's checkpoint is
 // restored and its steps re-run with every intermediate the adjoint reads
-// recorded per step (sorted state, permutation, offsets, neighbour masks,
+// recorded per step (sorted state, permutation, offsets, neighbour words,
 // densities, pressures, neighbour records, pre-foam force, heightfields,
 // column flux); the adjoint then walks the segment's steps in reverse.
 // Memory: checkpoints of the whole run plus the tape of one segment.
