@@ -291,6 +291,47 @@ def test_branch_scenes(gpu, cwa, oracle):
         assert np.isfinite(st["pos"]).all()
 
 
+@pytest.mark.parametrize("ratio", [2.2, 2.8])
+def test_dense_block_neighbour_paths(gpu, cwa, oracle, ratio):
+    """A compressed block (spacing h/2.2 and h/2.8: ~10 and ~22 particles per
+    cell, 50-90 neighbours each) drives the rarer neighbour-record paths of
+    the density / force passes: second words for candidates 32-63, columns
+    past 64 candidates and more non-zero words than a thread stages, whose
+    particles re-search their neighbours by distance.  One step against the
+    oracle, tolerances as the one-step test; the grid is bit-exact."""
+    from coupledwateranimation_amd import configs
+
+    base = configs.dam_break_small()
+    h = base.params.h
+    sp = h / ratio
+    n = 12 ** 3
+    pos = configs.lattice(n, (0.3, 0.2, 0.3), (0.3 + 12 * sp, 0.2 + 12 * sp, 0.3 + 12 * sp),
+                          sp, 0.5, 0.1, (2, 0, 1), 11)
+    vel = np.zeros_like(pos)
+    vel[:, 1] = -0.2
+    W = H = 32
+    h0 = np.zeros((H, W), np.float32)
+    _say(f"dense block ratio {ratio}: {n} particles")
+    s = cwa.Scene(base.params, pos, vel, None, (W, H), h0, h0)
+    s.step(base.dt, 1)
+    st = s.readback()
+    g = s.grid_host()
+    s.close()
+    o = oracle.OracleScene(base.params, pos, vel, None, (W, H), h0, h0)
+    o.step(base.dt, 1)
+    ref = o.state()
+    og = o.grid(base.params)
+    for k in ("key", "count", "offset", "list"):
+        assert np.array_equal(g[k], og[k]), k
+    if ratio > 2.5:
+        assert og["count"].max() * 3 > 64  # some 3-row column exceeds 64 candidates
+    assert og["count"].max() * 3 > 32
+    assert _rel(st["rho"], ref["rho"]) < 2e-6
+    assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
+    assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
+    assert _rel(st["vel"][:, :3], ref["vel"][:, :3]) < 2e-5
+
+
 def test_graph_equals_eager_and_deterministic(gpu, cwa):
     """hipGraph replay and eager launches give bit-identical states; two
     eager runs are bit-identical (no float atomics anywhere)."""
