@@ -146,6 +146,21 @@ __device__ __forceinline__ float wave_velocity(const Dev2& c, const Scene2D& S,
     return (wave_at(c, w0, S.L, sL, x) - wave_at(c, w1, S.L, sL, x)) / dts;
 }
 
+// Correctly rounded square root of x >= 0, as sqrtf: the hardware square
+// root (within 1 ulp) followed by the two residual tests that round it --
+// the sequence the compiler emits for sqrtf, without its input scaling for
+// x < 2^-96 and its select for 0 and inf, which change nothing for
+// x in [2^-96, inf) and at 0.  Inputs in (0, 2^-96) take sqrtf itself.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    if (__builtin_expect(x < 0x1p-96f && x > 0.0f, 0)) return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1);
+    const float sp = __int_as_float(__float_as_int(s) + 1);
+    float r = fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r = fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
+}
+
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
     const float f = clampf_nan(floorf((p - lo) / H), -1.0f, (float)dim);
     return min(max((int)f, 0), dim - 1);
@@ -367,7 +382,7 @@ __global__ void __launch_bounds__(BS) k2_density(Dev2 P, float dt, DensityArgs A
                     const float4 pj = A.xvB[j];
                     const float dx = x - pj.x, dy = y - pj.y;
                     const float r2 = fmaf(dy, dy, dx * dx);
-                    if (r2 < P.HSQ) acc += W_cubic(P, sqrtf(r2));
+                    if (r2 < P.HSQ) acc += W_cubic(P, sqrt_rn(r2));
                 }
             }
             rho = S.mass * acc;
@@ -429,7 +444,7 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                 const float r2 = fmaf(dy, dy, dx * dx);
                 if (j != i && r2 < P.T && r2 > 0.0f) {
                     const float2 bj = A.aj[j];
-                    const float r = sqrtf(r2);
+                    const float r = sqrt_rn(r2);
                     const float wg = W_cubic_grad(P, r);
                     const float inv = 1.0f / r;
                     const float ux = dx * inv, uy = dy * inv;
