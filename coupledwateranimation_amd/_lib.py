@@ -191,6 +191,7 @@ SIGNATURES = {
     "cwa_scene_particles": (C.c_int, [C.c_void_p, C.POINTER(ParticleView)]),
     "cwa_scene_grid": (C.c_int, [C.c_void_p, C.POINTER(GridView)]),
     "cwa_scene_readback": (C.c_int, [C.c_void_p, C.POINTER(HostState)]),
+    "cwa_scene_neighbour_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "cwa_scan_exclusive_i32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "cwa_grid_build": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_float,
                                  C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,

@@ -611,6 +611,35 @@ __global__ void __launch_bounds__(256) k_force_integrate(
 }
 
 // ---------------------------------------------------------------------------
+// Neighbour count of every slot as the last density pass found it (the
+// particle itself included, as the density sum of rho_pres_comp.glsl:60-68
+// counts it): popcounts of its neighbour words, or the re-search by distance
+// for a particle whose words are flagged incomplete.  pos / offset: that
+// pass's sorted positions and cell offsets.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_neighbour_counts(const DevParams* __restrict__ Pd,
+                                                          const float4* __restrict__ pos,
+                                                          const int* __restrict__ offset,
+                                                          const uint2* __restrict__ nbw, int n,
+                                                          int* __restrict__ out) {
+    const DevParams& P = *Pd;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = 1;  // the particle itself
+    const uint2 e0 = nbw[i];
+    if (e0.y & kNbTail) {
+        for_each_candidate_neighbour(P, offset, pos, i, pos[i], [&](int) { ++k; });
+    } else {
+        for (int c = 0; c < 9; ++c) {
+            const uint2 e = c == 0 ? e0 : nbw[(size_t)c * n + i];
+            k += __popc(e.x);
+            if (e.y & kNbHi) k += __popc(nbw[(size_t)(9 + c) * n + i].x);
+        }
+    }
+    out[i] = k;
+}
+
+// ---------------------------------------------------------------------------
 // Particle -> wave flux per vertical column of grid cells.  Column (ix, iz)
 // is one contiguous sorted range; one wave64 sums it (lane-strided partial
 // sums, then a fixed xor tree: deterministic).  Output at [iz*nx + ix].
@@ -795,6 +824,12 @@ hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleAr
     return hipGetLastError();
 }
 
+hipError_t launch_neighbour_counts(const DevParams& P, const float4* pos, const int* offset,
+                                  const uint2* nbw, int n, int* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_neighbour_counts<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, pos, offset, nbw, n, out);
+    return hipGetLastError();
+}
 hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
                               float* colflux, hipStream_t st) {
     const long long ncol = (long long)dims[0] * dims[2];

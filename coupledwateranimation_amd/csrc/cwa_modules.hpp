@@ -274,6 +274,9 @@ class CoupledScene : public Module {
     void WaveReinitFromTexture(const float* tex, int tw, int th);
     void CheckFinite();
     void Readback(cwa_host_state* h);
+    // Neighbour count (self included) of every particle, original order, as
+    // the last step's density pass found it.
+    void NeighbourCounts(int32_t* out);
     hipStream_t stream() const { return st_; }
     const DevParams& devparams() const { return dp_; }
     // Host-side phase of the buffer rotations (wave triple buffer x grid
@@ -313,6 +316,7 @@ class CoupledScene : public Module {
     float graph_dt_ = 0.0f;
     int graph_phase_ = -1;
     float dt_ = 0.0f;
+    bool stepped_ = false;  // a step has run since the last (re)init
     DeviceBuffer<int> flag_;
 };
 

@@ -396,6 +396,7 @@ void CoupledScene::Reinit(const cwa_init& init) {
     validate_init(init);
     invalidate_graph();
     hip_check(hipStreamSynchronize(st_), "sync");
+    stepped_ = false;
     particles.Allocate(init.n);
     particles.Upload(init.pos, init.vel, init.force, st_);
     grid.Init(grid_desc(params_), init.n);
@@ -417,10 +418,11 @@ void CoupledScene::SetParams(const cwa_params& p) {
                               p.grid_min[0] != params_.grid_min[0] ||
                               p.grid_min[1] != params_.grid_min[1] ||
                               p.grid_min[2] != params_.grid_min[2] ||
-                              p.cell_size != params_.cell_size;
+                              p.cell_size != params_.cell_size || ydiv_of(p) != ydiv_of(params_);
     invalidate_graph();
     hip_check(hipStreamSynchronize(st_), "sync");
     params_ = p;
+    stepped_ = false;  // the last step's neighbour records used the old parameters
     if (grid_changed) {
         grid.Init(grid_desc(params_), particles.n());
         grid.ClearCounters(st_);
@@ -475,6 +477,9 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
     wave.Evolve(dp_, couple ? cflux : nullptr, dt, st_);
     mark(K_WAVE, 1);
     wave.PingPong();
+    // the scene's own neighbour records and sorted copy now describe this step
+    // (a taped step writes them into the tape instead)
+    stepped_ = n > 0 && !tape;
     if (params_.check_finite && !ev) CheckFinite();
 }
 
@@ -553,6 +558,23 @@ void CoupledScene::WaveReinitFromTexture(const float* tex, int tw, int th) {
     invalidate_graph();
     hip_check(hipStreamSynchronize(st_), "sync");
     wave.ReinitFromTexture(tex, tw, th, st_);
+}
+
+void CoupledScene::NeighbourCounts(int32_t* out) {
+    const int n = particles.n();
+    if (n == 0) return;
+    if (!stepped_)
+        throw Error(CWA_ESTATE,
+                    "neighbour counts exist after a step, until the next reinit or set_params");
+    DeviceBuffer<int> d(n);
+    hip_check(launch_neighbour_counts(dp_, particles.B.pos, grid.offset(), particles.nbw(), n,
+                                      d.get(), st_),
+              "neighbour counts");
+    hip_check(hipStreamSynchronize(st_), "sync");
+    std::vector<int> cnt(n), id(n);
+    hip_check(hipMemcpy(cnt.data(), d.get(), n * sizeof(int), hipMemcpyDeviceToHost), "counts");
+    hip_check(hipMemcpy(id.data(), particles.A.id, n * sizeof(int), hipMemcpyDeviceToHost), "id");
+    for (int s = 0; s < n; ++s) out[id[s]] = cnt[s];
 }
 
 void CoupledScene::Readback(cwa_host_state* h) {
@@ -898,6 +920,13 @@ int cwa_scene_grid(const cwa_scene* s, cwa_grid_view* v) {
         v->offset = G.offset();
         v->list = G.list();
         v->flux = s->impl->colflux.get();
+    });
+}
+
+int cwa_scene_neighbour_counts(cwa_scene* s, int32_t* counts) {
+    return guarded([&] {
+        CWA_REQUIRE(s && counts, "null argument");
+        s->impl->NeighbourCounts(counts);
     });
 }
 

@@ -200,6 +200,16 @@ class Scene:
         v = self.particles()
         return _hip().d2h(v.id, 4 * v.n, np.int32, v.n)
 
+    def neighbour_counts(self) -> np.ndarray:
+        """Neighbour count (self included) of every particle, original order,
+        as the last step's density pass found it."""
+        n = self.particles().n
+        out = np.zeros(n, np.int32)
+        if n:
+            L.check(self._lib.cwa_scene_neighbour_counts(self._h,
+                                                         out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
     def readback(self) -> dict:
         """State in original particle order plus the two newest heights."""
         v = self.particles()

@@ -240,6 +240,11 @@ void* cwa_scene_stream(const cwa_scene* s);
 int cwa_scene_particles(const cwa_scene* s, cwa_particle_view* v);
 int cwa_scene_grid(const cwa_scene* s, cwa_grid_view* v);
 int cwa_scene_readback(cwa_scene* s, cwa_host_state* h);
+/* Neighbour count of every particle (original order, n int32 on the host)
+ * as the last step's density pass found it: the j with |x_i - x_j| < h, the
+ * particle itself included, over which rho_pres_comp.glsl:60-68 sums.
+ * CWA_ESTATE before the first step and after a reinit or set_params. */
+int cwa_scene_neighbour_counts(cwa_scene* s, int32_t* counts);
 
 /* ---- standalone primitives on caller device buffers (stream may be NULL
  *      for the null stream) ---- */

@@ -291,6 +291,50 @@ def test_branch_scenes(gpu, cwa, oracle):
         assert np.isfinite(st["pos"]).all()
 
 
+def _dense_block(ratio):
+    from coupledwateranimation_amd import configs
+
+    base = configs.dam_break_small()
+    sp = base.params.h / ratio
+    pos = configs.lattice(12 ** 3, (0.3, 0.2, 0.3), (0.3 + 12 * sp, 0.2 + 12 * sp, 0.3 + 12 * sp),
+                          sp, 0.5, 0.1, (2, 0, 1), 11)
+    return base, pos
+
+
+@pytest.mark.parametrize("name", ["dam_break_4k", "reference", "dam_break_4k_ydiv4",
+                                  "dense_2.2", "dense_2.8"])
+def test_neighbour_counts_bit_exact(gpu, cwa, oracle, name):
+    """Neighbour count of every particle (self included) as the first
+    step's density pass found it, against the oracle's grid search on the
+    same positions; for the 4 096-particle scene also against brute-force
+    all pairs.  The dense blocks drive the second words, long columns and
+    the re-search by distance."""
+    if name.startswith("dense"):
+        base, pos = _dense_block(float(name.split("_")[1]))
+        params, W = base.params, 32
+        s = cwa.Scene(params, pos, None, None, (W, W), np.zeros((W, W), np.float32))
+        dt = base.dt
+    else:
+        spec = _specs(cwa)[name]
+        params, pos, dt = spec.params, spec.pos, spec.dt
+        s = spec.make_scene()
+    _say(f"neighbour counts {name}: {pos.shape[0]} particles")
+    with pytest.raises(cwa.CwaError) as e:
+        s.neighbour_counts()          # no step yet
+    assert e.value.code == -4
+    s.step(dt, 1)
+    got = s.neighbour_counts()
+    s.set_params(params)
+    with pytest.raises(cwa.CwaError):
+        s.neighbour_counts()          # stale after set_params
+    s.close()
+    ref = oracle.neighbor_counts(params, pos, all_pairs=False)
+    assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} particles differ"
+    if name == "dam_break_4k":
+        assert np.array_equal(got, oracle.neighbor_counts(params, pos, all_pairs=True))
+    _say(f"  counts: mean {got.mean():.1f}, max {got.max()}")
+
+
 @pytest.mark.parametrize("ratio", [2.2, 2.8])
 def test_dense_block_neighbour_paths(gpu, cwa, oracle, ratio):
     """A compressed block (spacing h/2.2 and h/2.8: ~10 and ~22 particles per
@@ -299,14 +343,8 @@ def test_dense_block_neighbour_paths(gpu, cwa, oracle, ratio):
     past 64 candidates and more non-zero words than a thread stages, whose
     particles re-search their neighbours by distance.  One step against the
     oracle, tolerances as the one-step test; the grid is bit-exact."""
-    from coupledwateranimation_amd import configs
-
-    base = configs.dam_break_small()
-    h = base.params.h
-    sp = h / ratio
-    n = 12 ** 3
-    pos = configs.lattice(n, (0.3, 0.2, 0.3), (0.3 + 12 * sp, 0.2 + 12 * sp, 0.3 + 12 * sp),
-                          sp, 0.5, 0.1, (2, 0, 1), 11)
+    base, pos = _dense_block(ratio)
+    n = pos.shape[0]
     vel = np.zeros_like(pos)
     vel[:, 1] = -0.2
     W = H = 32
@@ -380,7 +418,8 @@ def test_empty_and_edge_scenes(gpu, cwa):
 
 def test_large_scene_step(gpu, cwa, oracle):
     """1 048 576 particles + 2048^2 heightfield: one step against the oracle
-    (16 threads), then 10 more steps checked through size-independent
+    (16 threads; grid arrays and every particle's neighbour count bit-exact),
+    then 10 more steps checked through size-independent
     properties: ids stay a permutation, slots stay cell-sorted, all finite,
     particles inside the box."""
     from coupledwateranimation_amd import configs
@@ -391,11 +430,13 @@ def test_large_scene_step(gpu, cwa, oracle):
     s.step(spec.dt, 1)
     st = s.readback()
     g = s.grid_host()
+    nb = s.neighbour_counts()
     _say("large: oracle step")
     o, ref = _oracle_run(oracle, spec, 1, threads=16)
     og = o.grid(spec.params)
     for k in ("key", "count", "offset", "list"):
         assert np.array_equal(g[k], og[k]), k
+    assert np.array_equal(nb, oracle.neighbor_counts(spec.params, spec.pos, all_pairs=False))
     assert _rel(st["rho"], ref["rho"]) < 2e-6
     assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
     assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
