@@ -213,6 +213,7 @@ __device__ __forceinline__ float fetch_clamped(const float* __restrict__ img, in
 
 __device__ __forceinline__ float bilerp(const float* __restrict__ img, int w, int h, float x,
                                         float y) {
+#pragma clang fp contract(off)  // each product and sum rounded, as the oracle's sampler
     float fx = floorf(x), fy = floorf(y);
     const float a = x - fx, b = y - fy;
     fx = fminf(fmaxf(fx, -2.0f), (float)(w + 1));
@@ -229,6 +230,7 @@ __device__ __forceinline__ float bilerp(const float* __restrict__ img, int w, in
 
 __device__ __forceinline__ float tex_sample(const float* __restrict__ img, int w, int h, float u,
                                             float v) {
+#pragma clang fp contract(off)
     return bilerp(img, w, h, u * (float)w - 0.5f, v * (float)h - 0.5f);
 }
 

@@ -700,18 +700,63 @@ __global__ void __launch_bounds__(256) k_wave(const DevParams* __restrict__ Pd, 
 // the west / east neighbours of the quad.  Per texel the arithmetic is that
 // of k_wave, so both give identical results.
 __device__ __forceinline__ float wave_texel(const DevParams& P, float C0, float C1, float nn,
-                                            float ss, float ee, float ww, int i, int j,
-                                            const float* __restrict__ colflux, float dt) {
+                                            float ss, float ee, float ww, int i, bool flux,
+                                            float S, float dt) {
 #pragma clang fp contract(off)
     float o = P.wc0 * C0 + P.lx * (ee + ww) + P.lz * (nn + ss) - P.wc1 * C1;
     o *= P.atten;
-    if (colflux) {
-        const float gx = P.ax * (float)i + P.bx, gz = P.az * (float)j + P.bz;
-        const float S = bilerp(colflux, P.dims[0], P.dims[2], gx, gz);
-        o += P.kappa * dt * (S * P.inv_area);
-    }
+    if (flux) o += P.kappa * dt * (S * P.inv_area);
     if (P.wake && o > P.wake_thr && i == P.W / 2) o += P.wake_amt;
     return o;
+}
+
+// Column-flux source of texels i0..i0+3 of row j: bilerp of the column grid
+// at each texel's (gx, gz), with the same values and arithmetic as bilerp.
+// The four samples share gz and, when they fall in at most two adjacent x
+// cells (the column grid is much coarser than the texels), also their six
+// grid values, loaded once.
+__device__ __forceinline__ void colflux_quad(const DevParams& P, const float* __restrict__ cf,
+                                             int i0, int j, float S[4]) {
+#pragma clang fp contract(off)
+    const int w = P.dims[0], h = P.dims[2];
+    const float gz = P.az * (float)j + P.bz;
+    float a[4];
+    int ix[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float gx = P.ax * (float)(i0 + t) + P.bx;
+        float fx = floorf(gx);
+        a[t] = gx - fx;
+        fx = fminf(fmaxf(fx, -2.0f), (float)(w + 1));
+        ix[t] = (int)fx;
+    }
+    if (ix[3] - ix[0] > 1) {  // a grid finer than the texels: sample one by one
+#pragma unroll
+        for (int t = 0; t < 4; ++t) S[t] = bilerp(cf, w, h, P.ax * (float)(i0 + t) + P.bx, gz);
+        return;
+    }
+    float fy = floorf(gz);
+    const float b = gz - fy;
+    fy = fminf(fmaxf(fy, -2.0f), (float)(h + 1));
+    const int j0 = (int)fy;
+    const size_t ra = (size_t)min(max(j0, 0), h - 1) * w;
+    const size_t rb = (size_t)min(max(j0 + 1, 0), h - 1) * w;
+    float r0[3], r1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ik = min(max(ix[0] + k, 0), w - 1);
+        r0[k] = cf[ra + ik];
+        r1[k] = cf[rb + ik];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bool d = ix[t] != ix[0];
+        const float t00 = d ? r0[1] : r0[0], t10 = d ? r0[2] : r0[1];
+        const float t01 = d ? r1[1] : r1[0], t11 = d ? r1[2] : r1[1];
+        const float q0 = (1.0f - a[t]) * t00 + a[t] * t10;
+        const float q1 = (1.0f - a[t]) * t01 + a[t] * t11;
+        S[t] = (1.0f - b) * q0 + b * q1;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_wave4(const DevParams* __restrict__ Pd, const float* __restrict__ h0,
@@ -730,11 +775,14 @@ __global__ void __launch_bounds__(256) k_wave4(const DevParams* __restrict__ Pd,
     const float4 ss = *reinterpret_cast<const float4*>(h0 + (size_t)max(j - 1, 0) * W + i0);
     const float west = h0[row + max(i0 - 1, 0)];
     const float east = h0[row + min(i0 + 4, W - 1)];
+    const bool flux = colflux != nullptr;
+    float S[4] = {0.f, 0.f, 0.f, 0.f};
+    if (flux) colflux_quad(P, colflux, i0, j, S);
     float4 o;
-    o.x = wave_texel(P, c0.x, c1.x, nn.x, ss.x, c0.y, west, i0, j, colflux, dt);
-    o.y = wave_texel(P, c0.y, c1.y, nn.y, ss.y, c0.z, c0.x, i0 + 1, j, colflux, dt);
-    o.z = wave_texel(P, c0.z, c1.z, nn.z, ss.z, c0.w, c0.y, i0 + 2, j, colflux, dt);
-    o.w = wave_texel(P, c0.w, c1.w, nn.w, ss.w, east, c0.z, i0 + 3, j, colflux, dt);
+    o.x = wave_texel(P, c0.x, c1.x, nn.x, ss.x, c0.y, west, i0, flux, S[0], dt);
+    o.y = wave_texel(P, c0.y, c1.y, nn.y, ss.y, c0.z, c0.x, i0 + 1, flux, S[1], dt);
+    o.z = wave_texel(P, c0.z, c1.z, nn.z, ss.z, c0.w, c0.y, i0 + 2, flux, S[2], dt);
+    o.w = wave_texel(P, c0.w, c1.w, nn.w, ss.w, east, c0.z, i0 + 3, flux, S[3], dt);
     *reinterpret_cast<float4*>(out + row + i0) = o;
 }
 
