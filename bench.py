@@ -65,14 +65,17 @@ def parse():
     ap.add_argument("--eager", action="store_true", help="launch kernels without hipGraph")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps of the per-kernel event pass (0: the same K as timed)")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="0 disables the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=80,
+                    help="timed oracle steps of the CPU baseline, about 10 s on 16 host "
+                         "cores (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
     ap.add_argument("--adjoint-steps", type=int, default=256,
                     help="steps of the forward+adjoint measurement (0 disables it)")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="steps of the batched 2D measurement (0 disables it)")
-    ap.add_argument("--batched-cpu-steps", type=int, default=1,
-                    help="oracle steps of the batched CPU baseline (0 disables it)")
+    ap.add_argument("--batched-cpu-steps", type=int, default=16,
+                    help="oracle steps of the batched CPU baseline, about 10 s on 16 host "
+                         "cores (0 disables it)")
     ap.add_argument("--fit-iters", type=int, default=50,
                     help="gradient-descent iterations of the batched fit (0 disables it)")
     ap.add_argument("--fit-scenes", type=int, default=64)
