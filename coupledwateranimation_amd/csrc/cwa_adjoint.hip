@@ -32,11 +32,37 @@ namespace cwa {
 
 namespace {
 
+// atomicAdd(p + idx, v) for every lane executing the call, combined per run
+// of consecutive lanes with equal idx: the run's first lane adds the run's
+// sum with one atomic.  The lanes of a wave hold neighbouring sorted
+// particles, whose texture taps mostly coincide, so one contended address
+// takes a few atomics per wave instead of up to 64.  Safe in divergent code:
+// lanes not executing the call end runs.
+__device__ __forceinline__ void atomic_add_runs(float* p, uint32_t idx, float v) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long act = __ballot(1);
+    const uint32_t prev = __shfl_up(idx, 1, 64);
+    const bool head = lane == 0 || !((act >> (lane - 1)) & 1ull) || prev != idx;
+    const unsigned long long stop = __ballot(head) | ~act;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long after = stop & ~upto;
+    const int run_end = after ? __ffsll((long long)after) - 1 : 64;
+    float s = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const float t = __shfl_down(s, o, 64);
+        if (lane + o < run_end) s += t;
+    }
+    if (head && s != 0.0f) atomicAdd(p + idx, s);
+}
+
 // d(bilerp)/d(x, y) at continuous texel coordinates; scatters g times the
-// four bilinear weights into img_bar (if not null).
+// four bilinear weights into img_bar (if not null).  Coordinates and weights
+// are rounded as the forward sampler rounds them.
 __device__ __forceinline__ void bilerp_adj(const float* __restrict__ img, float* img_bar, int w,
                                            int h, float x, float y, float g, float& gx,
                                            float& gy) {
+#pragma clang fp contract(off)
     float fx = floorf(x), fy = floorf(y);
     const float a = x - fx, b = y - fy;
     fx = fminf(fmaxf(fx, -2.0f), (float)(w + 1));
@@ -48,17 +74,18 @@ __device__ __forceinline__ void bilerp_adj(const float* __restrict__ img, float*
     const float t01 = img[(size_t)jb * w + ia], t11 = img[(size_t)jb * w + ib];
     gx = g * ((1.0f - b) * (t10 - t00) + b * (t11 - t01));
     gy = g * ((1.0f - a) * (t01 - t00) + a * (t11 - t10));
-    if (img_bar && g != 0.0f) {
-        atomicAdd(&img_bar[(size_t)ja * w + ia], g * (1.0f - a) * (1.0f - b));
-        atomicAdd(&img_bar[(size_t)ja * w + ib], g * a * (1.0f - b));
-        atomicAdd(&img_bar[(size_t)jb * w + ia], g * (1.0f - a) * b);
-        atomicAdd(&img_bar[(size_t)jb * w + ib], g * a * b);
+    if (img_bar) {
+        atomic_add_runs(img_bar, (uint32_t)(ja * w + ia), g * (1.0f - a) * (1.0f - b));
+        atomic_add_runs(img_bar, (uint32_t)(ja * w + ib), g * a * (1.0f - b));
+        atomic_add_runs(img_bar, (uint32_t)(jb * w + ia), g * (1.0f - a) * b);
+        atomic_add_runs(img_bar, (uint32_t)(jb * w + ib), g * a * b);
     }
 }
 
 // Adjoint of texture(img, (u, v)): returns d/du, d/dv times g.
 __device__ __forceinline__ void tex_adj(const float* __restrict__ img, float* img_bar, int w,
                                         int h, float u, float v, float g, float& gu, float& gv) {
+#pragma clang fp contract(off)
     float gx, gy;
     bilerp_adj(img, img_bar, w, h, u * (float)w - 0.5f, v * (float)h - 0.5f, g, gx, gy);
     gu = gx * (float)w;
