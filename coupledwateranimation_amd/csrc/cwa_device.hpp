@@ -125,15 +125,18 @@ constexpr uint32_t kNbHi = 1u << 31;
 constexpr uint32_t kNbTail = 1u << 30;
 constexpr uint32_t kNbSlot = kNbTail - 1u;
 
-// Up to kNbStage non-zero words of one particle, staged per thread in LDS
-// (slot t of the thread at s[t * stride]) by stage_nb_words, then walked in
-// order by NbCursor: one flat loop over all columns, so a wave runs as many
-// iterations as its busiest lane has neighbours rather than the sum over
-// columns of each column's busiest lane.
-constexpr int kNbStage = 12;
+// The non-zero words of one particle (at most kNbEntries), staged per
+// thread in LDS (slot t of the thread at s[t * stride]) by stage_nb_words,
+// then walked in order by NbCursor: one flat loop over all columns, so a
+// wave runs as many iterations as its busiest lane has neighbours rather
+// than the sum over columns of each column's busiest lane.  Room for every
+// word also caps a 256-thread block at 36 KB of LDS, four blocks per CU,
+// which measured faster for the force pass than six (fewer waves gathering
+// neighbour records through one L1).
+constexpr int kNbStage = kNbEntries;
 
 // Number of words staged, or -1 when the particle must be re-searched
-// (a column past 64 candidates, or more than kNbStage non-zero words).
+// (a column past 64 candidates).
 __device__ __forceinline__ int stage_nb_words(const uint2* __restrict__ nbw, int n, int i,
                                               uint2* s, int stride) {
     uint2 e[9];
@@ -144,16 +147,10 @@ __device__ __forceinline__ int stage_nb_words(const uint2* __restrict__ nbw, int
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
         const uint32_t b = e[c].y & kNbSlot;
-        if (e[c].x) {
-            if (nw < kNbStage) s[nw * stride] = make_uint2(e[c].x, b);
-            ++nw;
-        }
-        if (e[c].y & kNbHi) {
-            if (nw < kNbStage) s[nw * stride] = nbw[(size_t)(9 + c) * n + i];
-            ++nw;
-        }
+        if (e[c].x) s[(nw++) * stride] = make_uint2(e[c].x, b);
+        if (e[c].y & kNbHi) s[(nw++) * stride] = nbw[(size_t)(9 + c) * n + i];
     }
-    return nw > kNbStage ? -1 : nw;
+    return nw;
 }
 
 struct NbCursor {
@@ -180,8 +177,8 @@ struct NbCursor {
 };
 
 // Neighbours of particle i (slot i, position pi) in candidate order by the
-// exact distance test, excluding i: the path for particles whose words
-// overflowed.
+// exact distance test, excluding i: the path for particles with a column
+// of more than 64 candidates.
 template <class F>
 __device__ __forceinline__ void for_each_candidate_neighbour(const DevParams& P,
                                                              const int* __restrict__ offset,

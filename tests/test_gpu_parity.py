@@ -339,9 +339,9 @@ def test_neighbour_counts_bit_exact(gpu, cwa, oracle, name):
 def test_dense_block_neighbour_paths(gpu, cwa, oracle, ratio):
     """A compressed block (spacing h/2.2 and h/2.8: ~10 and ~22 particles per
     cell, 50-90 neighbours each) drives the rarer neighbour-record paths of
-    the density / force passes: second words for candidates 32-63, columns
-    past 64 candidates and more non-zero words than a thread stages, whose
-    particles re-search their neighbours by distance.  One step against the
+    the density / force passes: second words for candidates 32-63 and
+    columns past 64 candidates, whose particles re-search their neighbours
+    by distance.  One step against the
     oracle, tolerances as the one-step test; the grid is bit-exact."""
     base, pos = _dense_block(ratio)
     n = pos.shape[0]
