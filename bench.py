@@ -97,7 +97,6 @@ KERNEL_KEYS = [
     ("cwa::k_density(", "plain", "density"), ("cwa::k_force_integrate(", "plain", "force_integrate"),
     ("cwa::k_hash_count(", "plain", "hash"), ("cwa::k_scatter_tmp(", "plain", "scatter"),
     ("cwa::k_stable_reorder(", "plain", "reorder"), ("cwa::k_wave4(", "plain", "wave"),
-    ("cwa::k_column_flux(", "plain", "column_flux"),
 ]
 
 

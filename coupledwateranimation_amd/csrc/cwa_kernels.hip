@@ -16,7 +16,8 @@
 //   rho_pres_comp.glsl                               -> k_density
 //   force_comp.glsl + integrate_comp.glsl            -> k_force_integrate
 //   wave_comp.glsl (EVOLVE, INIT)                    -> k_wave, k_init_wave
-//   particle -> heightfield flux (no reference)      -> k_column_flux
+//   particle -> heightfield flux (no reference)      -> column blocks of
+//                                                       k_force_integrate
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -468,10 +469,34 @@ __device__ __forceinline__ void force_pair(float h, float4 pi, float4 vi, float 
 }
 
 // ---------------------------------------------------------------------------
+// Particle -> wave flux per vertical column of grid cells.  Column (ix, iz)
+// is one contiguous sorted range; one wave64 sums it (lane-strided partial
+// sums, then a fixed xor tree: deterministic).  Output at [iz*nx + ix].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void column_flux_sum(const int* __restrict__ offset,
+                                                const float* __restrict__ flux, int nx, int ny,
+                                                int nz, float* __restrict__ colflux, int wave,
+                                                int lane) {
+    if (wave >= nx * nz) return;  // uniform across the wave
+    const int b = offset[wave * ny], e = offset[(wave + 1) * ny];
+    float s = 0.0f;
+    for (int k = b + lane; k < e; k += 64) s += flux[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        const int ix = wave / nz, iz = wave - ix * nz;
+        colflux[iz * nx + ix] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Forces (force_comp.glsl:62-139) fused with symplectic Euler + boundary
 // (integrate_comp.glsl:56-92, 135-178).  Visits only the neighbours the
 // density pass recorded (its neighbour words, in candidate order), reads
 // the sorted copy B and writes the state A in the same slot order.
+// Blocks nfb.. of the grid sum the particle flux per column instead (one
+// wave per column; they read only the density pass's output and run in
+// the force blocks' tail).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_force_integrate(
     const DevParams* __restrict__ Pd, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -480,9 +505,16 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     const uint2* __restrict__ nbw, const float4* __restrict__ nrec,
     const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
-    float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre) {
+    float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre, int nfb,
+    const float* __restrict__ pflux, float* __restrict__ colflux) {
     const DevParams& P = *Pd;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if ((int)blockIdx.x >= nfb) {
+        column_flux_sum(offset, pflux, P.dims[0], P.dims[1], P.dims[2], colflux,
+                        (((int)blockIdx.x - nfb) * (int)blockDim.x + (int)threadIdx.x) >> 6,
+                        threadIdx.x & 63);
+        return;
+    }
+    const int i = xcd_block(blockIdx.x, nfb) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
     const float4 vi = vel[i];
@@ -637,29 +669,6 @@ __global__ void __launch_bounds__(256) k_neighbour_counts(const DevParams* __res
         }
     }
     out[i] = k;
-}
-
-// ---------------------------------------------------------------------------
-// Particle -> wave flux per vertical column of grid cells.  Column (ix, iz)
-// is one contiguous sorted range; one wave64 sums it (lane-strided partial
-// sums, then a fixed xor tree: deterministic).  Output at [iz*nx + ix].
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_column_flux(const int* __restrict__ offset,
-                                                     const float* __restrict__ flux, int nx,
-                                                     int ny, int nz, float* __restrict__ colflux) {
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    const int ncol = nx * nz;
-    if (wave >= ncol) return;
-    const int b = offset[wave * ny], e = offset[(wave + 1) * ny];
-    float s = 0.0f;
-    for (int k = b + lane; k < e; k += 64) s += flux[k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) {
-        const int ix = wave / nz, iz = wave - ix * nz;
-        colflux[iz * nx + ix] = s;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -863,12 +872,14 @@ hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleAr
                                   const int* offset, int n, const float* rho_w, const float2* pw,
                                   const uint2* nbw, const float4* nrec, const float* H0,
                                   ParticleArrays& A, float* rho, float* prs, float4* fpre,
-                                  hipStream_t st) {
+                                  const float* flux, float* colflux, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_force_integrate<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, dt, B.pos, B.vel, B.force, B.id,
-                                                          offset, n, rho_w, pw, nbw, nrec, H0,
-                                                          A.pos, A.vel, A.force, A.id, rho, prs,
-                                                          fpre);
+    const int nfb = blocks_for(n, 256);
+    const long long ncol = colflux ? (long long)P.dims[0] * P.dims[2] : 0;
+    const int ncb = blocks_for(ncol * 64, 256);
+    k_force_integrate<<<nfb + ncb, 256, 0, st>>>(P.dev, dt, B.pos, B.vel, B.force, B.id, offset, n,
+                                                 rho_w, pw, nbw, nrec, H0, A.pos, A.vel, A.force,
+                                                 A.id, rho, prs, fpre, nfb, flux, colflux);
     return hipGetLastError();
 }
 
@@ -876,14 +887,6 @@ hipError_t launch_neighbour_counts(const DevParams& P, const float4* pos, const 
                                   const uint2* nbw, int n, int* out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_neighbour_counts<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, pos, offset, nbw, n, out);
-    return hipGetLastError();
-}
-hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
-                              float* colflux, hipStream_t st) {
-    const long long ncol = (long long)dims[0] * dims[2];
-    if (ncol <= 0) return hipSuccess;
-    k_column_flux<<<blocks_for(ncol * 64, 256), 256, 0, st>>>(offset, flux, dims[0], dims[1],
-                                                              dims[2], colflux);
     return hipGetLastError();
 }
 

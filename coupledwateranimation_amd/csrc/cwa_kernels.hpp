@@ -47,12 +47,10 @@ hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleAr
                                   const int* offset, int n, const float* rho_w, const float2* pw,
                                   const uint2* nbw, const float4* nrec, const float* H0,
                                   ParticleArrays& A, float* rho, float* prs, float4* fpre,
-                                  hipStream_t st);
+                                  const float* flux, float* colflux, hipStream_t st);
 // Neighbour counts (self included) of the last density pass, per sorted slot.
 hipError_t launch_neighbour_counts(const DevParams& P, const float4* pos, const int* offset,
                                    const uint2* nbw, int n, int* out, hipStream_t st);
-hipError_t launch_column_flux(const int* offset, const float* flux, const int dims[3],
-                              float* colflux, hipStream_t st);
 hipError_t launch_wave(const DevParams& P, const float* h0, const float* h1, float* out,
                        const float* colflux, float dt, hipStream_t st);
 hipError_t launch_init_wave(float* out, int W, int H, float type, hipStream_t st);

@@ -460,18 +460,14 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
                                  tape ? tape->rho_raw : nullptr, st_),
                   "density");
         mark(K_DENSITY, 1);
+        // the same launch sums the particle flux per column when coupled
         mark(K_FORCE, 0);
         hip_check(launch_force_integrate(dp_, dt, B, off, n, rho_w, pw, nbm, nrec, H0,
                                          particles.A, particles.rho(), particles.prs(),
-                                         tape ? tape->fpre : nullptr, st_),
+                                         tape ? tape->fpre : nullptr, particles.flux(),
+                                         couple ? cflux : nullptr, st_),
                   "force/integrate");
         mark(K_FORCE, 1);
-        if (couple) {
-            mark(K_COLFLUX, 0);
-            hip_check(launch_column_flux(off, particles.flux(), params_.grid_dims, cflux, st_),
-                      "column flux");
-            mark(K_COLFLUX, 1);
-        }
     }
     mark(K_WAVE, 0);
     wave.Evolve(dp_, couple ? cflux : nullptr, dt, st_);
@@ -864,14 +860,12 @@ int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times
             }
         } cleanup{ev};
         CoupledScene& S = *s->impl;
-        const bool couple = S.particles.n() > 0 && S.params().couple_band > 0.0f;
         for (int k = 0; k < nsteps; ++k) {
             S.Step(dt, ev);
             hip_check(hipStreamSynchronize(S.stream()), "sync");
             for (int slot = 0; slot < K_NUM; ++slot) {
                 const bool ran = slot == K_WAVE ||
-                                 (S.particles.n() > 0 && slot >= K_HASH && slot <= K_FORCE) ||
-                                 (slot == K_COLFLUX && couple);
+                                 (S.particles.n() > 0 && slot >= K_HASH && slot <= K_FORCE);
                 if (!ran) continue;
                 float ms = 0.0f;
                 hip_check(hipEventElapsedTime(&ms, ev[slot][0], ev[slot][1]), "elapsed");

@@ -169,7 +169,8 @@ typedef struct cwa_host_state {
 } cwa_host_state;
 
 /* Average per-kernel device time over a profiled run (HIP events on the
- * scene stream). */
+ * scene stream).  Slots that did not run have launches 0 (the column flux
+ * is summed inside the force launch and is timed with it). */
 #define CWA_NUM_KERNELS 10
 typedef struct cwa_kernel_times {
     int32_t steps;
@@ -221,8 +222,8 @@ int cwa_scene_height(const cwa_scene* s, int age, float** dptr, int* w, int* h, 
 
 /* ---- the coupled step: idle() Main.cpp:530-562 ----
  * Per step: grid build (hash, count, scan, stable insert + reorder),
- * density/pressure, forces + symplectic Euler + boundary, particle ->
- * wave column flux, wave EVOLVE + PingPong. */
+ * density/pressure, forces + symplectic Euler + boundary with the particle
+ * -> wave column flux in the same launch, wave EVOLVE + PingPong. */
 int cwa_scene_step(cwa_scene* s, float dt, int nsteps);
 /* Same as cwa_scene_step with HIP events around every kernel; `out`
  * receives the per-kernel average device times. */
