@@ -123,6 +123,32 @@ def test_grid_build_bit_exact_1m(gpu, cwa, oracle):
         assert np.array_equal(g[k], r[k]), k
 
 
+def test_scene_grid_over_2m_cells(gpu, cwa, oracle):
+    """A scene grid of 3.9 M cells (more than 2^21) takes the cell scan's
+    recursive path (tile sums scanned by a nested scan).  Three steps use and
+    re-clear both halves of the double-buffered cell counters; each step's
+    grid arrays equal the oracle's grid build on the same slot-ordered
+    positions."""
+    from coupledwateranimation_amd import configs
+
+    spec = configs.dam_break_small()
+    p = spec.params
+    configs.grid_for_box(p, (-2.0, -2.0, -2.0), (3.0, 3.0, 3.0), p.h, 2)
+    nc = p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2]
+    assert nc > 2 ** 21
+    _say(f"scene grid of {nc} cells")
+    s = spec.make_scene()
+    for k in range(3):
+        st = s.readback()
+        pos_slots = st["pos"][s.slot_ids()]
+        s.step(spec.dt, 1)
+        g = s.grid_host()
+        r = oracle.grid_build(p, pos_slots)
+        for key in ("key", "count", "offset", "list"):
+            assert np.array_equal(g[key], r[key]), (k, key)
+    s.close()
+
+
 # ----------------------------------------------------------------- wave
 def _wave_params(cwa, wtype=1.0):
     p = cwa.reference_params()
