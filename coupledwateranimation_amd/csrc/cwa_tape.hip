@@ -8,7 +8,11 @@
 // recorded per step (sorted state, permutation, offsets, neighbour words,
 // densities, pressures, neighbour records, pre-foam force, heightfields,
 // column flux); the adjoint then walks the segment's steps in reverse.
-// Memory: checkpoints of the whole run plus the tape of one segment.
+// The eager Backward pipelines the segments: the scene's stream re-runs
+// segment s-1 into a second tape while a second stream runs the adjoint of
+// segment s (BackwardSeeded, which a graph captures, keeps one stream).
+// Memory: checkpoints of the whole run plus the tapes of one segment (two
+// for the eager Backward).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -24,6 +28,9 @@ namespace cwa {
 class Tape {
    public:
     Tape(CoupledScene& s, int nsteps, int every);
+    ~Tape();
+    Tape(const Tape&) = delete;
+    Tape& operator=(const Tape&) = delete;
     void Forward(float dt);
     void Backward(const float* dH0, const float* dH1, const float* dpos, const float* dvel,
                   double* grad, float* dpos0, float* dvel0);
@@ -52,10 +59,12 @@ class Tape {
         DeviceBuffer<float> rho, rho_raw, h0, h1, colflux;
         DeviceBuffer<float2> pw;
     };
+    void alloc_step_tape(StepTape& t);
     void save(Ckpt& c);
     void restore(const Ckpt& c);
     void record_step(StepTape& t);
-    void adjoint_step(const StepTape& t);
+    void adjoint_step(const StepTape& t, hipStream_t st);
+    void reduce_partials(hipStream_t st);
     template <class T>
     void copy(T* dst, const T* src, size_t count) {
         if (count)
@@ -70,7 +79,9 @@ class Tape {
     float dt_ = 0.0f;
     bool forward_done_ = false;
     std::vector<Ckpt> ck_;
-    std::vector<StepTape> tp_;
+    std::vector<StepTape> tp_, tp2_;  // tp2_: the eager Backward's second tape
+    hipStream_t ast_ = nullptr;       // the eager Backward's adjoint stream
+    hipEvent_t ev_[5] = {};           // recorded[2], adjoint done[2], setup / end
     DeviceBuffer<float4> xb_, vb_, xb2_, vb2_, xl_, vl_, adjrec_;
     DeviceBuffer<float> G0_, G1_, G0n_, G1n_, rhob_, rtb_, Sbar_;
     DeviceBuffer<double> pgrad_, acc_wave_, acc_pair_;
@@ -102,21 +113,7 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
         c.h1.resize(tex);
     }
     tp_.resize(std::min(every, nsteps));
-    for (auto& t : tp_) {
-        t.pos.resize(n);
-        t.vel.resize(n);
-        t.nrec.resize(2 * n);
-        t.fpre.resize(n);
-        t.list.resize(n);
-        t.offset.resize((size_t)ncell_ + 1);
-        t.nbw.resize((size_t)kNbEntries * n);
-        t.rho.resize(n);
-        t.rho_raw.resize(n);
-        t.pw.resize(n);
-        t.h0.resize(tex);
-        t.h1.resize(tex);
-        t.colflux.resize(ncol_);
-    }
+    for (auto& t : tp_) alloc_step_tape(t);
     for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
     adjrec_.resize(2 * n);
     for (auto* b : {&G0_, &G1_, &G0n_, &G1n_}) b->resize(tex);
@@ -126,6 +123,32 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
     pgrad_.resize(GRAD_NUM);
     acc_wave_.resize((size_t)adj_wave_blocks(s.devparams()) * 5);
     acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
+}
+
+Tape::~Tape() {
+    if (ast_) {
+        (void)hipStreamSynchronize(ast_);
+        (void)hipStreamDestroy(ast_);
+    }
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+void Tape::alloc_step_tape(StepTape& t) {
+    const size_t n = (size_t)std::max(n_, 1), tex = (size_t)W_ * H_;
+    t.pos.resize(n);
+    t.vel.resize(n);
+    t.nrec.resize(2 * n);
+    t.fpre.resize(n);
+    t.list.resize(n);
+    t.offset.resize((size_t)ncell_ + 1);
+    t.nbw.resize((size_t)kNbEntries * n);
+    t.rho.resize(n);
+    t.rho_raw.resize(n);
+    t.pw.resize(n);
+    t.h0.resize(tex);
+    t.h1.resize(tex);
+    t.colflux.resize(ncol_);
 }
 
 void Tape::save(Ckpt& c) {
@@ -195,17 +218,21 @@ void Tape::BackwardSeeded() {
         restore(ck_[seg]);
         const int len = std::min(every_, nsteps_ - seg * every_);
         for (int k = 0; k < len; ++k) record_step(tp_[k]);
-        for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k]);
+        for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k], st_);
     }
+    reduce_partials(st_);
+    restore(ck_[nseg_]);
+}
+
+void Tape::reduce_partials(hipStream_t st) {
     const int wslots[5] = {GRAD_LAMBDA_X, GRAD_LAMBDA_Z, GRAD_ATTEN, GRAD_BETA, GRAD_KAPPA};
     const int pslots[3] = {GRAD_VISC, GRAD_GAS_CONST, GRAD_REST_RHO};
     hip_check(launch_reduce_partials(acc_wave_.get(), (int)(acc_wave_.size() / 5), 5, wslots,
-                                     pgrad_.get(), st_),
+                                     pgrad_.get(), st),
               "reduce");
     hip_check(launch_reduce_partials(acc_pair_.get(), (int)(acc_pair_.size() / 3), 3, pslots,
-                                     pgrad_.get(), st_),
+                                     pgrad_.get(), st),
               "reduce");
-    restore(ck_[nseg_]);
 }
 
 void Tape::Forward(float dt) {
@@ -244,28 +271,28 @@ void Tape::record_step(StepTape& t) {
     S_.Step(dt_, nullptr, &out);
 }
 
-void Tape::adjoint_step(const StepTape& t) {
+void Tape::adjoint_step(const StepTape& t, hipStream_t st) {
     const DevParams& P = S_.devparams();
     const bool couple = n_ > 0 && P.band > 0.0f;
     hip_check(launch_adj_wave(P, dt_, t.h0.get(), t.h1.get(), couple ? t.colflux.get() : nullptr,
-                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(), st_),
+                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(), st),
               "adj wave");
     if (n_ > 0) {
         if (couple)
-            hip_check(launch_adj_colflux(P, dt_, G0_.get(), Sbar_.get(), st_), "adj colflux");
+            hip_check(launch_adj_colflux(P, dt_, G0_.get(), Sbar_.get(), st), "adj colflux");
         hip_check(launch_adj_local(P, dt_, n_, t.pos.get(), t.vel.get(), t.fpre.get(),
                                    t.rho.get(), t.h0.get(), xb_.get(), vb_.get(), G0n_.get(),
-                                   adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st_),
+                                   adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st),
                   "adj local");
         hip_check(launch_adj_pairs(P, dt_, n_, t.pos.get(), t.vel.get(), t.offset.get(),
                                    t.nbw.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
                                    t.rho_raw.get(), t.h0.get(), Sbar_.get(), adjrec_.get(),
                                    rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
-                                   acc_pair_.get(), st_),
+                                   acc_pair_.get(), st),
                   "adj pairs");
         hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbw.get(),
                                        rtb_.get(), xl_.get(), vl_.get(), t.list.get(),
-                                       xb2_.get(), vb2_.get(), st_),
+                                       xb2_.get(), vb2_.get(), st),
                   "adj rho pairs");
         xb_.swap(xb2_);
         vb_.swap(vb2_);
@@ -310,20 +337,34 @@ void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const
     hip_check(hipMemsetAsync(acc_wave_.get(), 0, acc_wave_.size() * sizeof(double), st_), "acc");
     hip_check(hipMemsetAsync(acc_pair_.get(), 0, acc_pair_.size() * sizeof(double), st_), "acc");
 
-    for (int seg = nseg_ - 1; seg >= 0; --seg) {
+    // Pipelined sweep: segment s is re-run on the scene's stream into tape
+    // set s % 2 while the adjoint stream walks the previous segment's set;
+    // a set is re-recorded only after the adjoint that read it is done.
+    if (!ast_) {
+        hip_check(hipStreamCreateWithFlags(&ast_, hipStreamNonBlocking), "adjoint stream");
+        for (hipEvent_t& e : ev_)
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    }
+    if (nseg_ > 1 && tp2_.size() != tp_.size()) {
+        tp2_.resize(tp_.size());
+        for (auto& t : tp2_) alloc_step_tape(t);
+    }
+    hip_check(hipEventRecord(ev_[4], st_), "event");
+    hip_check(hipStreamWaitEvent(ast_, ev_[4], 0), "wait");
+    for (int q = 0, seg = nseg_ - 1; seg >= 0; ++q, --seg) {
+        std::vector<StepTape>& set = (q & 1) ? tp2_ : tp_;
+        if (q >= 2) hip_check(hipStreamWaitEvent(st_, ev_[2 + (q & 1)], 0), "wait");
         restore(ck_[seg]);
         const int len = std::min(every_, nsteps_ - seg * every_);
-        for (int k = 0; k < len; ++k) record_step(tp_[k]);
-        for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k]);
+        for (int k = 0; k < len; ++k) record_step(set[k]);
+        hip_check(hipEventRecord(ev_[q & 1], st_), "event");
+        hip_check(hipStreamWaitEvent(ast_, ev_[q & 1], 0), "wait");
+        for (int k = len - 1; k >= 0; --k) adjoint_step(set[k], ast_);
+        hip_check(hipEventRecord(ev_[2 + (q & 1)], ast_), "event");
     }
-    const int wslots[5] = {GRAD_LAMBDA_X, GRAD_LAMBDA_Z, GRAD_ATTEN, GRAD_BETA, GRAD_KAPPA};
-    const int pslots[3] = {GRAD_VISC, GRAD_GAS_CONST, GRAD_REST_RHO};
-    hip_check(launch_reduce_partials(acc_wave_.get(), (int)(acc_wave_.size() / 5), 5, wslots,
-                                     pgrad_.get(), st_),
-              "reduce");
-    hip_check(launch_reduce_partials(acc_pair_.get(), (int)(acc_pair_.size() / 3), 3, pslots,
-                                     pgrad_.get(), st_),
-              "reduce");
+    reduce_partials(ast_);
+    hip_check(hipEventRecord(ev_[4], ast_), "event");
+    hip_check(hipStreamWaitEvent(st_, ev_[4], 0), "wait");
     restore(ck_[nseg_]);  // leave the scene in its final state
     hip_check(hipStreamSynchronize(st_), "backward");
 
