@@ -292,8 +292,9 @@ int cwa_tape_forward(cwa_tape* t, float dt);
  * (w*h, newest / previous heightfield), dpos / dvel (4*n, original particle
  * order, xyz used).  Writes the parameter gradients and, if not NULL, the
  * gradients with respect to the initial positions / velocities (4*n,
- * original order).  Recomputes each segment from its checkpoint; the scene
- * is left in the final state. */
+ * original order).  Recomputes each segment from its checkpoint (on the
+ * scene stream, overlapped with the previous segment's adjoint on a second
+ * stream the tape owns); the scene is left in the final state. */
 int cwa_tape_backward(cwa_tape* t, const float* dheight0, const float* dheight1,
                       const float* dpos, const float* dvel, cwa_grad* grad, float* dpos0,
                       float* dvel0);
