@@ -235,6 +235,12 @@ struct TapeOut {
     float* rho = nullptr;      // density (after the clamp)
     float2* pw = nullptr;      // (pressure, 1/rho)
     float* colflux = nullptr;  // column flux
+    // heightfields: when h_out is set the step reads h_in0 (newest) and
+    // h_in1 (previous) instead of the scene's triple buffer and writes the
+    // new heightfield to h_out (the scene's buffer is then left stale)
+    const float* h_in0 = nullptr;
+    const float* h_in1 = nullptr;
+    float* h_out = nullptr;
 };
 
 // Kernel slots of cwa_kernel_times.

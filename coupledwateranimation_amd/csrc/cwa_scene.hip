@@ -438,7 +438,8 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
         if (ev) hip_check(hipEventRecord(ev[slot][which], st_), "event");
     };
     const int n = particles.n();
-    const float* H0 = wave.GetReadImage(0);
+    const bool taped_heights = tape && tape->h_out;
+    const float* H0 = taped_heights ? tape->h_in0 : wave.GetReadImage(0);
     const bool couple = n > 0 && params_.couple_band > 0.0f;
     // where this step's arrays live (the scene's own, or the tape's)
     auto pick = [&](auto* t, auto* own) { return (tape && t) ? t : own; };
@@ -470,13 +471,19 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
         mark(K_FORCE, 1);
     }
     mark(K_WAVE, 0);
-    wave.Evolve(dp_, couple ? cflux : nullptr, dt, st_);
+    if (taped_heights)
+        hip_check(launch_wave(dp_, tape->h_in0, tape->h_in1, tape->h_out, couple ? cflux : nullptr,
+                              dt, st_),
+                  "wave");
+    else
+        wave.Evolve(dp_, couple ? cflux : nullptr, dt, st_);
     mark(K_WAVE, 1);
     wave.PingPong();
     // the scene's own neighbour records and sorted copy now describe this step
     // (a taped step writes them into the tape instead)
     stepped_ = n > 0 && !tape;
-    if (params_.check_finite && !ev) CheckFinite();
+    // a taped step replays one the forward already checked
+    if (params_.check_finite && !ev && !tape) CheckFinite();
 }
 
 void CoupledScene::CheckFinite() {

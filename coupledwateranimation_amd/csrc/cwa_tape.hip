@@ -27,7 +27,9 @@ namespace cwa {
 
 class Tape {
    public:
-    Tape(CoupledScene& s, int nsteps, int every);
+    // pipelined: allocate the second step tape, stream and events of the
+    // eager Backward up front (tapes that only run BackwardSeeded pass false)
+    Tape(CoupledScene& s, int nsteps, int every, bool pipelined);
     ~Tape();
     Tape(const Tape&) = delete;
     Tape& operator=(const Tape&) = delete;
@@ -56,13 +58,17 @@ class Tape {
         DeviceBuffer<float4> pos, vel, nrec, fpre;
         DeviceBuffer<int> list, offset;
         DeviceBuffer<uint2> nbw;
-        DeviceBuffer<float> rho, rho_raw, h0, h1, colflux;
+        DeviceBuffer<float> rho, rho_raw, hout, colflux;
         DeviceBuffer<float2> pw;
+        // the two heightfields the step read: the segment's checkpoint or
+        // earlier steps' hout (set while recording)
+        const float* h0 = nullptr;
+        const float* h1 = nullptr;
     };
     void alloc_step_tape(StepTape& t);
     void save(Ckpt& c);
-    void restore(const Ckpt& c);
-    void record_step(StepTape& t);
+    void restore(const Ckpt& c, bool heights = true);
+    void record_step(std::vector<StepTape>& set, int k, const Ckpt& c);
     void adjoint_step(const StepTape& t, hipStream_t st);
     void reduce_partials(hipStream_t st);
     template <class T>
@@ -87,7 +93,8 @@ class Tape {
     DeviceBuffer<double> pgrad_, acc_wave_, acc_pair_;
 };
 
-Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
+Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
+    : S_(s), st_(s.stream()) {
     if (nsteps < 1) throw Error(CWA_EINVAL, "nsteps must be >= 1");
     if (every < 1) throw Error(CWA_EINVAL, "ckpt_every must be >= 1");
     if (s.params().torque_coeff != 0.0f)
@@ -114,6 +121,15 @@ Tape::Tape(CoupledScene& s, int nsteps, int every) : S_(s), st_(s.stream()) {
     }
     tp_.resize(std::min(every, nsteps));
     for (auto& t : tp_) alloc_step_tape(t);
+    if (pipelined) {
+        if (nseg_ > 1) {
+            tp2_.resize(tp_.size());
+            for (auto& t : tp2_) alloc_step_tape(t);
+        }
+        hip_check(hipStreamCreateWithFlags(&ast_, hipStreamNonBlocking), "adjoint stream");
+        for (hipEvent_t& e : ev_)
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    }
     for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
     adjrec_.resize(2 * n);
     for (auto* b : {&G0_, &G1_, &G0n_, &G1n_}) b->resize(tex);
@@ -146,8 +162,7 @@ void Tape::alloc_step_tape(StepTape& t) {
     t.rho.resize(n);
     t.rho_raw.resize(n);
     t.pw.resize(n);
-    t.h0.resize(tex);
-    t.h1.resize(tex);
+    t.hout.resize(tex);
     t.colflux.resize(ncol_);
 }
 
@@ -163,7 +178,7 @@ void Tape::save(Ckpt& c) {
     copy(c.h1.get(), S_.wave.GetReadImage(1), (size_t)W_ * H_);
 }
 
-void Tape::restore(const Ckpt& c) {
+void Tape::restore(const Ckpt& c, bool heights) {
     auto& A = S_.particles.A;
     copy(A.pos, c.pos.get(), n_);
     copy(A.vel, c.vel.get(), n_);
@@ -171,8 +186,10 @@ void Tape::restore(const Ckpt& c) {
     copy(A.id, c.id.get(), n_);
     copy(S_.particles.rho(), c.rho.get(), n_);
     copy(S_.particles.prs(), c.prs.get(), n_);
-    copy(S_.wave.GetReadImage(0), c.h0.get(), (size_t)W_ * H_);
-    copy(S_.wave.GetReadImage(1), c.h1.get(), (size_t)W_ * H_);
+    if (heights) {
+        copy(S_.wave.GetReadImage(0), c.h0.get(), (size_t)W_ * H_);
+        copy(S_.wave.GetReadImage(1), c.h1.get(), (size_t)W_ * H_);
+    }
     S_.grid.ClearCounters(st_);
 }
 
@@ -215,9 +232,9 @@ void Tape::BackwardSeeded() {
     zero(acc_wave_);
     zero(acc_pair_);
     for (int seg = nseg_ - 1; seg >= 0; --seg) {
-        restore(ck_[seg]);
+        restore(ck_[seg], false);
         const int len = std::min(every_, nsteps_ - seg * every_);
-        for (int k = 0; k < len; ++k) record_step(tp_[k]);
+        for (int k = 0; k < len; ++k) record_step(tp_, k, ck_[seg]);
         for (int k = len - 1; k >= 0; --k) adjoint_step(tp_[k], st_);
     }
     reduce_partials(st_);
@@ -249,14 +266,19 @@ void Tape::Forward(float dt) {
     forward_done_ = true;
 }
 
-// One step of the segment replay, recording what the adjoint reads: the
-// step writes its intermediates straight into the tape slot; only the two
-// heightfields it reads are copied (the triple buffer reuses them).
-void Tape::record_step(StepTape& t) {
-    const size_t tex = (size_t)W_ * H_;
-    copy(t.h0.get(), S_.wave.GetReadImage(0), tex);
-    copy(t.h1.get(), S_.wave.GetReadImage(1), tex);
+// Step k of the segment replay from checkpoint c, recording what the
+// adjoint reads: the step writes its intermediates straight into the tape
+// slot, reads its two heightfields from the checkpoint (k = 0, 1) or the
+// earlier slots' outputs, and writes the new heightfield into its own slot,
+// so no heightfield is copied.
+void Tape::record_step(std::vector<StepTape>& set, int k, const Ckpt& c) {
+    StepTape& t = set[k];
+    t.h0 = k == 0 ? c.h0.get() : set[k - 1].hout.get();
+    t.h1 = k == 0 ? c.h1.get() : (k == 1 ? c.h0.get() : set[k - 2].hout.get());
     TapeOut out;
+    out.h_in0 = t.h0;
+    out.h_in1 = t.h1;
+    out.h_out = t.hout.get();
     out.rho_raw = t.rho_raw.get();
     out.fpre = t.fpre.get();
     out.pos = t.pos.get();
@@ -274,19 +296,19 @@ void Tape::record_step(StepTape& t) {
 void Tape::adjoint_step(const StepTape& t, hipStream_t st) {
     const DevParams& P = S_.devparams();
     const bool couple = n_ > 0 && P.band > 0.0f;
-    hip_check(launch_adj_wave(P, dt_, t.h0.get(), t.h1.get(), couple ? t.colflux.get() : nullptr,
+    hip_check(launch_adj_wave(P, dt_, t.h0, t.h1, couple ? t.colflux.get() : nullptr,
                               G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(), st),
               "adj wave");
     if (n_ > 0) {
         if (couple)
             hip_check(launch_adj_colflux(P, dt_, G0_.get(), Sbar_.get(), st), "adj colflux");
         hip_check(launch_adj_local(P, dt_, n_, t.pos.get(), t.vel.get(), t.fpre.get(),
-                                   t.rho.get(), t.h0.get(), xb_.get(), vb_.get(), G0n_.get(),
+                                   t.rho.get(), t.h0, xb_.get(), vb_.get(), G0n_.get(),
                                    adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st),
                   "adj local");
         hip_check(launch_adj_pairs(P, dt_, n_, t.pos.get(), t.vel.get(), t.offset.get(),
                                    t.nbw.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
-                                   t.rho_raw.get(), t.h0.get(), Sbar_.get(), adjrec_.get(),
+                                   t.rho_raw.get(), t.h0, Sbar_.get(), adjrec_.get(),
                                    rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
                                    acc_pair_.get(), st),
                   "adj pairs");
@@ -340,23 +362,15 @@ void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const
     // Pipelined sweep: segment s is re-run on the scene's stream into tape
     // set s % 2 while the adjoint stream walks the previous segment's set;
     // a set is re-recorded only after the adjoint that read it is done.
-    if (!ast_) {
-        hip_check(hipStreamCreateWithFlags(&ast_, hipStreamNonBlocking), "adjoint stream");
-        for (hipEvent_t& e : ev_)
-            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-    }
-    if (nseg_ > 1 && tp2_.size() != tp_.size()) {
-        tp2_.resize(tp_.size());
-        for (auto& t : tp2_) alloc_step_tape(t);
-    }
+    if (!ast_) throw Error(CWA_ESTATE, "tape created without the eager backward's resources");
     hip_check(hipEventRecord(ev_[4], st_), "event");
     hip_check(hipStreamWaitEvent(ast_, ev_[4], 0), "wait");
     for (int q = 0, seg = nseg_ - 1; seg >= 0; ++q, --seg) {
         std::vector<StepTape>& set = (q & 1) ? tp2_ : tp_;
         if (q >= 2) hip_check(hipStreamWaitEvent(st_, ev_[2 + (q & 1)], 0), "wait");
-        restore(ck_[seg]);
+        restore(ck_[seg], false);
         const int len = std::min(every_, nsteps_ - seg * every_);
-        for (int k = 0; k < len; ++k) record_step(set[k]);
+        for (int k = 0; k < len; ++k) record_step(set, k, ck_[seg]);
         hip_check(hipEventRecord(ev_[q & 1], st_), "event");
         hip_check(hipStreamWaitEvent(ast_, ev_[q & 1], 0), "wait");
         for (int k = len - 1; k >= 0; --k) adjoint_step(set[k], ast_);
@@ -432,7 +446,7 @@ class FitBatch {
         for (int s = 0; s < ns; ++s) {
             size_t f0 = 0, f1 = 0, tot = 0;
             hip_check(hipMemGetInfo(&f0, &tot), "mem info");
-            tapes_[s].reset(new Tape(*sc_[s], nsteps, every));
+            tapes_[s].reset(new Tape(*sc_[s], nsteps, every, false));
             const int tex = tapes_[s]->texels();
             target_[s].resize(tex);
             hip_check(hipMemcpy(target_[s].get(), targets[s], sizeof(float) * tex,
@@ -569,7 +583,7 @@ int cwa_tape_create(cwa_scene* s, int nsteps, int ckpt_every, cwa_tape** out) {
         *out = nullptr;
         auto* t = new cwa_tape{nullptr};
         try {
-            t->impl = new Tape(*scene_impl(s), nsteps, ckpt_every);
+            t->impl = new Tape(*scene_impl(s), nsteps, ckpt_every, true);
         } catch (...) {
             delete t;
             throw;
