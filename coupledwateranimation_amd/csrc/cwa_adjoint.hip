@@ -10,9 +10,9 @@
 // One reverse step, in sorted order of the forward step:
 //   k_adj_wave        texel gather: transpose of the 5-point stencil, the
 //                     pass-through of h^n into h^(n-1), and the partials of
-//                     lambda_x, lambda_z, atten, beta, kappa
-//   k_adj_colflux     column gather: transpose of the bilinear column-flux
-//                     source
+//                     lambda_x, lambda_z, atten, beta, kappa; its extra
+//                     block rows: column gather, the transpose of the
+//                     bilinear column-flux source
 //   k_adj_local       per particle: integrate, boundary, surface clamp,
 //                     gravity, drag, wave force, crest / viscosity scale
 //   k_adj_pairs       neighbour gather: pressure and viscosity pair terms
@@ -168,10 +168,56 @@ struct AdjNbr {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Column-flux adjoint: Sbar[col] = kappa dt / cell^2 * sum over texels t of
+// G0[t] * (bilinear weight of col in t's source sample).  One wave per
+// column visits the texels whose sample can touch it (texel -> column map
+// is affine), in a fixed order.  Runs as extra blocks of the wave adjoint's
+// launch (both read only G0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void adj_colflux_column(const DevParams& P, float dt,
+                                                   const float* __restrict__ G0,
+                                                   float* __restrict__ Sbar, int col, int lane) {
+    // one wave64 per column; lanes stride over the column's texel window
+    const int nx = P.dims[0], nz = P.dims[2];
+    if (col >= nx * nz) return;  // uniform across the wave
+    const int cx = col % nx, cz = col / nx;  // Sbar / colflux layout [iz*nx + ix]
+    const int W = P.W, H = P.H;
+    // texel range whose gx lies in (cx - 1, cx + 1), widened by one texel;
+    // edge columns also collect the texels clamped onto them
+    int ilo = 0, ihi = W - 1, jlo = 0, jhi = H - 1;
+    if (cx > 0) ilo = max(0, (int)floorf(((float)cx - 1.0f - P.bx) / P.ax) - 1);
+    if (cx < nx - 1) ihi = min(W - 1, (int)ceilf(((float)cx + 1.0f - P.bx) / P.ax) + 1);
+    if (cz > 0) jlo = max(0, (int)floorf(((float)cz - 1.0f - P.bz) / P.az) - 1);
+    if (cz < nz - 1) jhi = min(H - 1, (int)ceilf(((float)cz + 1.0f - P.bz) / P.az) + 1);
+    const int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
+    const long long total = (ni > 0 && nj > 0) ? (long long)ni * nj : 0;
+    float acc = 0.0f;
+    for (long long t = lane; t < total; t += 64) {
+        const int j = jlo + (int)(t / ni), i = ilo + (int)(t % ni);
+        const float gz = P.az * (float)j + P.bz;
+        float fz = floorf(gz);
+        const float bz = gz - fz;
+        fz = fminf(fmaxf(fz, -2.0f), (float)(nz + 1));
+        const int z0 = min(max((int)fz, 0), nz - 1), z1 = min(max((int)fz + 1, 0), nz - 1);
+        const float wz = (z0 == cz ? 1.0f - bz : 0.0f) + (z1 == cz ? bz : 0.0f);
+        const float gx = P.ax * (float)i + P.bx;
+        float fx = floorf(gx);
+        const float bx = gx - fx;
+        fx = fminf(fmaxf(fx, -2.0f), (float)(nx + 1));
+        const int x0 = min(max((int)fx, 0), nx - 1), x1 = min(max((int)fx + 1, 0), nx - 1);
+        const float wx = (x0 == cx ? 1.0f - bx : 0.0f) + (x1 == cx ? bx : 0.0f);
+        acc += G0[(size_t)j * W + i] * (wx * wz);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) Sbar[col] = P.kappa * dt * P.inv_area * acc;
+}
+
+// ---------------------------------------------------------------------------
 // Wave adjoint.  G0 = adjoint of h^(n+1) (the step's output), G1 = adjoint of
 // h^n carried by the pass-through into h^(n-1) of the next step.  Writes
 // G0n = adjoint of h^n (before particle-sample contributions) and
-// G1n = adjoint of h^(n-1).
+// G1n = adjoint of h^(n-1).  Block rows nby.. compute the column-flux
+// adjoint Sbar instead (when Sbar is not null).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_adj_wave(const DevParams* __restrict__ Pd, float dt,
                                                   const float* __restrict__ H0,
@@ -181,8 +227,15 @@ __global__ void __launch_bounds__(256) k_adj_wave(const DevParams* __restrict__ 
                                                   const float* __restrict__ G1,
                                                   float* __restrict__ G0n,
                                                   float* __restrict__ G1n,
-                                                  double* __restrict__ acc) {
+                                                  double* __restrict__ acc, int nby,
+                                                  float* __restrict__ Sbar) {
     const DevParams& P = *Pd;
+    if ((int)blockIdx.y >= nby) {  // column-flux adjoint blocks
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        const int b = ((int)blockIdx.y - nby) * (int)gridDim.x + (int)blockIdx.x;
+        adj_colflux_column(P, dt, G0, Sbar, (b * 256 + tid) >> 6, tid & 63);
+        return;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = P.W, H = P.H;
@@ -221,53 +274,6 @@ __global__ void __launch_bounds__(256) k_adj_wave(const DevParams* __restrict__ 
         }
     }
     block_accumulate<5>(part, acc);  // lambda_x, lambda_z, atten, beta, kappa
-}
-
-// ---------------------------------------------------------------------------
-// Column-flux adjoint: Sbar[col] = kappa dt / cell^2 * sum over texels t of
-// G0[t] * (bilinear weight of col in t's source sample).  One thread per
-// column visits the texels whose sample can touch it (texel -> column map
-// is affine), in a fixed order.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_colflux(const DevParams* __restrict__ Pd, float dt,
-                                                     const float* __restrict__ G0,
-                                                     float* __restrict__ Sbar) {
-    const DevParams& P = *Pd;
-    // one wave64 per column; lanes stride over the column's texel window
-    const int col = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    const int nx = P.dims[0], nz = P.dims[2];
-    if (col >= nx * nz) return;  // uniform across the wave
-    const int cx = col % nx, cz = col / nx;  // Sbar / colflux layout [iz*nx + ix]
-    const int W = P.W, H = P.H;
-    // texel range whose gx lies in (cx - 1, cx + 1), widened by one texel;
-    // edge columns also collect the texels clamped onto them
-    int ilo = 0, ihi = W - 1, jlo = 0, jhi = H - 1;
-    if (cx > 0) ilo = max(0, (int)floorf(((float)cx - 1.0f - P.bx) / P.ax) - 1);
-    if (cx < nx - 1) ihi = min(W - 1, (int)ceilf(((float)cx + 1.0f - P.bx) / P.ax) + 1);
-    if (cz > 0) jlo = max(0, (int)floorf(((float)cz - 1.0f - P.bz) / P.az) - 1);
-    if (cz < nz - 1) jhi = min(H - 1, (int)ceilf(((float)cz + 1.0f - P.bz) / P.az) + 1);
-    const int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
-    const long long total = (ni > 0 && nj > 0) ? (long long)ni * nj : 0;
-    float acc = 0.0f;
-    for (long long t = lane; t < total; t += 64) {
-        const int j = jlo + (int)(t / ni), i = ilo + (int)(t % ni);
-        const float gz = P.az * (float)j + P.bz;
-        float fz = floorf(gz);
-        const float bz = gz - fz;
-        fz = fminf(fmaxf(fz, -2.0f), (float)(nz + 1));
-        const int z0 = min(max((int)fz, 0), nz - 1), z1 = min(max((int)fz + 1, 0), nz - 1);
-        const float wz = (z0 == cz ? 1.0f - bz : 0.0f) + (z1 == cz ? bz : 0.0f);
-        const float gx = P.ax * (float)i + P.bx;
-        float fx = floorf(gx);
-        const float bx = gx - fx;
-        fx = fminf(fmaxf(fx, -2.0f), (float)(nx + 1));
-        const int x0 = min(max((int)fx, 0), nx - 1), x1 = min(max((int)fx + 1, 0), nx - 1);
-        const float wx = (x0 == cx ? 1.0f - bx : 0.0f) + (x1 == cx ? bx : 0.0f);
-        acc += G0[(size_t)j * W + i] * (wx * wz);
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) Sbar[col] = P.kappa * dt * P.inv_area * acc;
 }
 
 // Sum the per-block parameter partials (np slots per block, nb blocks) in
@@ -628,16 +634,15 @@ hipError_t launch_mse_seed(const float* h, const float* target, int N, float* G,
 
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,
                            const float* colflux, const float* G0, const float* G1, float* G0n,
-                           float* G1n, double* acc, hipStream_t st) {
-    dim3 blk(64, 4), grd((P.W + 63) / 64, (P.H + 3) / 4);
-    k_adj_wave<<<grd, blk, 0, st>>>(P.dev, dt, H0, H1, colflux, G0, G1, G0n, G1n, acc);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_colflux(const DevParams& P, float dt, const float* G0, float* Sbar,
-                              hipStream_t st) {
-    const long long ncol = (long long)P.dims[0] * P.dims[2];
-    k_adj_colflux<<<nblk(ncol * 64, 256), 256, 0, st>>>(P.dev, dt, G0, Sbar);
+                           float* G1n, double* acc, float* Sbar, hipStream_t st) {
+    const int gx = (P.W + 63) / 64, nby = (P.H + 3) / 4;
+    int rows = 0;
+    if (Sbar) {
+        const long long ncb = nblk((long long)P.dims[0] * P.dims[2] * 64, 256);
+        rows = (int)((ncb + gx - 1) / gx);
+    }
+    dim3 blk(64, 4), grd(gx, nby + rows);
+    k_adj_wave<<<grd, blk, 0, st>>>(P.dev, dt, H0, H1, colflux, G0, G1, G0n, G1n, acc, nby, Sbar);
     return hipGetLastError();
 }
 

@@ -29,11 +29,10 @@ hipError_t launch_mse_seed(const float* h, const float* target, int N, float* G,
                            int nblocks, double* loss, hipStream_t st);
 hipError_t launch_reduce_partials(const double* acc, int nb, int np, const int* slots,
                                   double* pgrad, hipStream_t st);
+// Sbar (optional): the same launch also writes the column-flux adjoint.
 hipError_t launch_adj_wave(const DevParams& P, float dt, const float* H0, const float* H1,
                            const float* colflux, const float* G0, const float* G1, float* G0n,
-                           float* G1n, double* acc, hipStream_t st);
-hipError_t launch_adj_colflux(const DevParams& P, float dt, const float* G0, float* Sbar,
-                              hipStream_t st);
+                           float* G1n, double* acc, float* Sbar, hipStream_t st);
 hipError_t launch_adj_local(const DevParams& P, float dt, int n, const float4* pos,
                             const float4* vel, const float4* fpre, const float* rho_w,
                             const float* H0, const float4* xb_in, const float4* vb_in,

@@ -297,11 +297,10 @@ void Tape::adjoint_step(const StepTape& t, hipStream_t st) {
     const DevParams& P = S_.devparams();
     const bool couple = n_ > 0 && P.band > 0.0f;
     hip_check(launch_adj_wave(P, dt_, t.h0, t.h1, couple ? t.colflux.get() : nullptr,
-                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(), st),
-              "adj wave");
+                              G0_.get(), G1_.get(), G0n_.get(), G1n_.get(), acc_wave_.get(),
+                              couple ? Sbar_.get() : nullptr, st),
+              "adj wave + column flux");
     if (n_ > 0) {
-        if (couple)
-            hip_check(launch_adj_colflux(P, dt_, G0_.get(), Sbar_.get(), st), "adj colflux");
         hip_check(launch_adj_local(P, dt_, n_, t.pos.get(), t.vel.get(), t.fpre.get(),
                                    t.rho.get(), t.h0, xb_.get(), vb_.get(), G0n_.get(),
                                    adjrec_.get(), rhob_.get(), xl_.get(), vl_.get(), st),
