@@ -56,11 +56,92 @@ __device__ __forceinline__ void atomic_add_runs(float* p, uint32_t idx, float v)
     if (head && s != 0.0f) atomicAdd(p + idx, s);
 }
 
+// Where a heightfield-sample adjoint adds its four weighted taps.
+// TexScatterGlobal: run-aggregated global atomics into img_bar.
+struct TexScatterGlobal {
+    float* img_bar;
+    int w;
+    __device__ __forceinline__ void operator()(int ia, int ja, float v) const {
+        atomic_add_runs(img_bar, (uint32_t)(ja * w + ia), v);
+    }
+};
+
+// TexScatterWindow: the particles of a block are neighbours in sorted order,
+// so most of their taps fall in one small patch of texels, and the taps one
+// texture width away (the wave normal's H(u + 1, v), H(u, v + 1)) clamp onto
+// the same short stretch of the last column or row.  Taps inside the block's
+// kTexWin x kTexWin LDS window (origin ox, oy), or on the last column / row
+// beside it, accumulate in LDS with LDS atomics, and tex_window_end adds
+// them to img_bar once per block; the other taps take run-aggregated global
+// atomics.
+constexpr int kTexWin = 64;
+constexpr int kTexWinLds = kTexWin * kTexWin + 2 * kTexWin;  // window, last column, last row
+struct TexScatterWindow {
+    float* img_bar;
+    int w, h;
+    float* win;  // [kTexWin^2] window, [kTexWin] last column, [kTexWin] last row
+    int ox, oy;
+    __device__ __forceinline__ void operator()(int ia, int ja, float v) const {
+        const unsigned dx = (unsigned)(ia - ox), dy = (unsigned)(ja - oy);
+        const unsigned n = (unsigned)kTexWin;
+        if (dx < n && dy < n)
+            atomicAdd(&win[dy * n + dx], v);
+        else if (ia == w - 1 && dy < n)
+            atomicAdd(&win[n * n + dy], v);
+        else if (ja == h - 1 && dx < n)
+            atomicAdd(&win[n * n + n + dx], v);
+        else
+            atomic_add_runs(img_bar, (uint32_t)(ja * w + ia), v);
+    }
+};
+
+// Window of the block: origin kTexWin/2 texels before the texel of the
+// block's first slot (which exists for every launched block), cleared.
+// Called by every thread of the block.
+__device__ __forceinline__ TexScatterWindow tex_window_begin(const DevParams& P,
+                                                             const float4* __restrict__ pos,
+                                                             float* img_bar, float* win,
+                                                             int* org) {
+    if (threadIdx.x == 0) {
+        const float4 x0 = pos[(size_t)blockIdx.x * blockDim.x];
+        const float u = (x0.x - P.org[0]) * P.uvs[0] * (float)P.W - 0.5f;
+        const float v = (x0.z - P.org[1]) * P.uvs[1] * (float)P.H - 0.5f;
+        org[0] = (int)floorf(fminf(fmaxf(u, -1e6f), 1e6f)) - kTexWin / 2;
+        org[1] = (int)floorf(fminf(fmaxf(v, -1e6f), 1e6f)) - kTexWin / 2;
+    }
+    for (int t = threadIdx.x; t < kTexWinLds; t += blockDim.x) win[t] = 0.0f;
+    __syncthreads();
+    return TexScatterWindow{img_bar, P.W, P.H, win, org[0], org[1]};
+}
+
+// Add the block's LDS accumulators to img_bar (non-zero texels inside the
+// image).  Called by every thread of the block.
+__device__ __forceinline__ void tex_window_end(const DevParams& P, const TexScatterWindow& sc) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTexWinLds; t += blockDim.x) {
+        const float v = sc.win[t];
+        int tx, ty;
+        if (t < kTexWin * kTexWin) {
+            tx = sc.ox + t % kTexWin;
+            ty = sc.oy + t / kTexWin;
+        } else if (t < kTexWin * kTexWin + kTexWin) {
+            tx = P.W - 1;
+            ty = sc.oy + (t - kTexWin * kTexWin);
+        } else {
+            tx = sc.ox + (t - kTexWin * kTexWin - kTexWin);
+            ty = P.H - 1;
+        }
+        if (v != 0.0f && tx >= 0 && tx < P.W && ty >= 0 && ty < P.H)
+            atomicAdd(&sc.img_bar[(size_t)ty * P.W + tx], v);
+    }
+}
+
 // d(bilerp)/d(x, y) at continuous texel coordinates; scatters g times the
-// four bilinear weights into img_bar (if not null).  Coordinates and weights
-// are rounded as the forward sampler rounds them.
-__device__ __forceinline__ void bilerp_adj(const float* __restrict__ img, float* img_bar, int w,
-                                           int h, float x, float y, float g, float& gx,
+// four bilinear weights through `add` (if not null).  Coordinates and
+// weights are rounded as the forward sampler rounds them.
+template <class Scatter>
+__device__ __forceinline__ void bilerp_adj(const float* __restrict__ img, const Scatter* add,
+                                           int w, int h, float x, float y, float g, float& gx,
                                            float& gy) {
 #pragma clang fp contract(off)
     float fx = floorf(x), fy = floorf(y);
@@ -74,20 +155,21 @@ __device__ __forceinline__ void bilerp_adj(const float* __restrict__ img, float*
     const float t01 = img[(size_t)jb * w + ia], t11 = img[(size_t)jb * w + ib];
     gx = g * ((1.0f - b) * (t10 - t00) + b * (t11 - t01));
     gy = g * ((1.0f - a) * (t01 - t00) + a * (t11 - t10));
-    if (img_bar) {
-        atomic_add_runs(img_bar, (uint32_t)(ja * w + ia), g * (1.0f - a) * (1.0f - b));
-        atomic_add_runs(img_bar, (uint32_t)(ja * w + ib), g * a * (1.0f - b));
-        atomic_add_runs(img_bar, (uint32_t)(jb * w + ia), g * (1.0f - a) * b);
-        atomic_add_runs(img_bar, (uint32_t)(jb * w + ib), g * a * b);
+    if (add) {
+        (*add)(ia, ja, g * (1.0f - a) * (1.0f - b));
+        (*add)(ib, ja, g * a * (1.0f - b));
+        (*add)(ia, jb, g * (1.0f - a) * b);
+        (*add)(ib, jb, g * a * b);
     }
 }
 
 // Adjoint of texture(img, (u, v)): returns d/du, d/dv times g.
-__device__ __forceinline__ void tex_adj(const float* __restrict__ img, float* img_bar, int w,
+template <class Scatter>
+__device__ __forceinline__ void tex_adj(const float* __restrict__ img, const Scatter* add, int w,
                                         int h, float u, float v, float g, float& gu, float& gv) {
 #pragma clang fp contract(off)
     float gx, gy;
-    bilerp_adj(img, img_bar, w, h, u * (float)w - 0.5f, v * (float)h - 0.5f, g, gx, gy);
+    bilerp_adj(img, add, w, h, u * (float)w - 0.5f, v * (float)h - 0.5f, g, gx, gy);
     gu = gx * (float)w;
     gv = gy * (float)h;
 }
@@ -294,24 +376,17 @@ __global__ void k_reduce_partials(const double* __restrict__ acc, int nb, int np
 // Per-particle adjoint of integrate + the force terms local to the particle.
 // xb, vb: adjoints of the step outputs (sorted order of this step).
 // Writes F-bar and V-bar (the viscosity sum's adjoint, w = crest factor),
-// the direct rho-bar, and the local parts of x-bar / v-bar.
+// the direct rho-bar, and the local parts of x-bar / v-bar.  Heightfield
+// taps go through the block's LDS window.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__ Pd, float dt, int n,
-                                                   const float4* __restrict__ pos,
-                                                   const float4* __restrict__ vel,
-                                                   const float4* __restrict__ fpre,
-                                                   const float* __restrict__ rho_w,
-                                                   const float* __restrict__ H0,
-                                                   const float4* __restrict__ xb_in,
-                                                   const float4* __restrict__ vb_in,
-                                                   float* __restrict__ G0n,
-                                                   float4* __restrict__ adjrec,
-                                                   float* __restrict__ rhob_d,
-                                                   float4* __restrict__ xb_loc,
-                                                   float4* __restrict__ vb_loc) {
-    const DevParams& P = *Pd;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// The per-particle body of k_adj_local; texture taps go through `sc`.
+__device__ __forceinline__ void adj_local_particle(
+    const DevParams& P, float dt, int i, const float4* __restrict__ pos,
+    const float4* __restrict__ vel, const float4* __restrict__ fpre,
+    const float* __restrict__ rho_w, const float* __restrict__ H0,
+    const float4* __restrict__ xb_in, const float4* __restrict__ vb_in,
+    const TexScatterWindow& sc, float4* __restrict__ adjrec, float* __restrict__ rhob_d,
+    float4* __restrict__ xb_loc, float4* __restrict__ vb_loc) {
     const float4 x = pos[i], v = vel[i], F = fpre[i];
     const float rho = rho_w[i];
     // --- forward recompute of integrate (branch decisions) ---
@@ -348,7 +423,7 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
         const float thb = xby;
         xby = 0.0f;
         float gu, gv;
-        tex_adj(H0, G0n, P.W, P.H, tu, tv, thb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, tu, tv, thb, gu, gv);
         xbx += gu * P.uvs[0];
         xbz += gv * P.uvs[1];
     }
@@ -384,19 +459,19 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
         const float nxb = -P.wforce * Hs * Fbx, nyb = -P.wforce * Hs * Fby;
         hb -= nxb + nyb;
         float gu, gv, su = 0.0f, sv = 0.0f;
-        tex_adj(H0, G0n, P.W, P.H, u + P.du, vv, hxb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, u + P.du, vv, hxb, gu, gv);
         su += gu;
         sv += gv;
-        tex_adj(H0, G0n, P.W, P.H, u, vv + P.du, hyb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, u, vv + P.du, hyb, gu, gv);
         su += gu;
         sv += gv;
-        tex_adj(H0, G0n, P.W, P.H, u + 1.0f, vv, nxb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, u + 1.0f, vv, nxb, gu, gv);
         su += gu;
         sv += gv;
-        tex_adj(H0, G0n, P.W, P.H, u, vv + 1.0f, nyb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, u, vv + 1.0f, nyb, gu, gv);
         su += gu;
         sv += gv;
-        tex_adj(H0, G0n, P.W, P.H, u, vv, hb, gu, gv);
+        tex_adj(H0, &sc, P.W, P.H, u, vv, hb, gu, gv);
         su += gu;
         sv += gv;
         lxbx += su * P.uvs[0];
@@ -409,6 +484,29 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
     rhob_d[i] = rhob;
     xb_loc[i] = make_float4(lxbx, lxby, lxbz, 0.0f);
     vb_loc[i] = make_float4(lvbx, lvby, lvbz, 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__ Pd, float dt, int n,
+                                                   const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel,
+                                                   const float4* __restrict__ fpre,
+                                                   const float* __restrict__ rho_w,
+                                                   const float* __restrict__ H0,
+                                                   const float4* __restrict__ xb_in,
+                                                   const float4* __restrict__ vb_in,
+                                                   float* __restrict__ G0n,
+                                                   float4* __restrict__ adjrec,
+                                                   float* __restrict__ rhob_d,
+                                                   float4* __restrict__ xb_loc,
+                                                   float4* __restrict__ vb_loc) {
+    const DevParams& P = *Pd;
+    __shared__ float s_win[kTexWinLds];
+    __shared__ int s_org[2];
+    const TexScatterWindow sc = tex_window_begin(P, pos, G0n, s_win, s_org);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adj_local_particle(P, dt, i, pos, vel, fpre, rho_w, H0, xb_in, vb_in, sc, adjrec,
+                                  rhob_d, xb_loc, vb_loc);
+    tex_window_end(P, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -535,7 +633,8 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
         }
         if (Hb != 0.0f) {
             float gu, gv;
-            tex_adj(H0, G0n, P.W, P.H, u, vv, Hb, gu, gv);
+            const TexScatterGlobal sc{G0n, P.W};
+            tex_adj(H0, &sc, P.W, P.H, u, vv, Hb, gu, gv);
             xbx += gu * P.uvs[0];
             xbz += gv * P.uvs[1];
         }
