@@ -344,6 +344,44 @@ __device__ __forceinline__ void density_candidate(const float4 pi, const float4 
     word |= min(__float_as_uint(w), 1u) << k;
 }
 
+// What both density kernels do with a particle's kernel sum: EOS, wave
+// term, rest-density clamp, the neighbour record and the particle flux.
+__device__ __forceinline__ void density_finish(const DevParams& P, int i, float4 pi, float4 vi,
+                                               float acc, const float* __restrict__ H0,
+                                               float* __restrict__ rho_w,
+                                               float2* __restrict__ pw, float* __restrict__ flux,
+                                               float4* __restrict__ nrec,
+                                               float* __restrict__ rho_raw) {
+    float rho = P.m_poly6 * acc;
+    if (rho_raw) rho_raw[i] = rho;  // taped for the adjoint
+    float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
+    const float u = (pi.x - P.org[0]) * P.uvs[0];
+    const float v = (pi.z - P.org[1]) * P.uvs[1];
+    const float height = tex_sample(H0, P.W, P.H, u, v);
+    if (coupled(P, pi.y, height)) {
+        const float wf = height * rho;
+        pressure += wf;
+        rho += wf / P.k_rp;
+    }
+    rho = fmaxf(P.rho0, rho);
+    const float irho = 1.0f / rho;
+    rho_w[i] = rho;
+    pw[i] = make_float2(pressure, irho);
+    // the 32-byte record the force pass reads for this particle as a neighbour
+    nrec[2 * (size_t)i] = make_float4(pi.x, pi.y, pi.z, pressure);
+    nrec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, irho);
+    float fl = 0.0f;
+    if (P.band > 0.0f) {
+        const float d = fabsf(pi.y - height);
+        if (d < P.band) {
+            const float qq = d / P.band;
+            const float w = (1.0f - qq * qq) * (1.0f - qq * qq);
+            fl = P.m_rho0 * vi.y * w;
+        }
+    }
+    flux[i] = fl;
+}
+
 // ---------------------------------------------------------------------------
 // Density / pressure (rho_pres_comp.glsl:49-81) + particle flux term.
 // Also records the exact neighbour set of every particle as neighbour words
@@ -402,34 +440,65 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
         if (hi) nbw[(size_t)(9 + c) * n + i] = make_uint2(hi, (uint32_t)b + 32u);
     }
     nbw[i] = make_uint2(e0.x, e0.y | (tail ? kNbTail : 0u));
-    float rho = P.m_poly6 * acc;
-    if (rho_raw) rho_raw[i] = rho;  // taped for the adjoint
-    float pressure = fmaxf(P.k * (rho - P.rho0), 0.0f);
-    const float u = (pi.x - P.org[0]) * P.uvs[0];
-    const float v = (pi.z - P.org[1]) * P.uvs[1];
-    const float height = tex_sample(H0, P.W, P.H, u, v);
-    if (coupled(P, pi.y, height)) {
-        const float wf = height * rho;
-        pressure += wf;
-        rho += wf / P.k_rp;
-    }
-    rho = fmaxf(P.rho0, rho);
-    const float irho = 1.0f / rho;
-    rho_w[i] = rho;
-    pw[i] = make_float2(pressure, irho);
-    // the 32-byte record the force pass reads for this particle as a neighbour
-    nrec[2 * (size_t)i] = make_float4(pi.x, pi.y, pi.z, pressure);
-    nrec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, irho);
-    float fl = 0.0f;
-    if (P.band > 0.0f) {
-        const float d = fabsf(pi.y - height);
-        if (d < P.band) {
-            const float qq = d / P.band;
-            const float w = (1.0f - qq * qq) * (1.0f - qq * qq);
-            fl = P.m_rho0 * vi.y * w;
+    density_finish(P, i, pi, vi, acc, H0, rho_w, pw, flux, nrec, rho_raw);
+}
+
+// Density for scenes of up to kDensitySplitMaxN particles: kDensitySplit
+// consecutive lanes per particle, each testing every kDensitySplit-th
+// candidate of each column, so a scene of some ten thousand particles still
+// keeps several waves per SIMD busy.  The group's neighbour bits are OR-ed
+// together (the words equal k_density's), its partial sums added in a fixed
+// tree, and the group's first lane finishes the particle.
+constexpr int kDensitySplit = 4;
+constexpr int kDensitySplitMaxN = 1 << 18;
+__global__ void __launch_bounds__(256) k_density_split(
+    const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
+    const float4* __restrict__ vel, const int* __restrict__ offset, int n,
+    const float* __restrict__ H0, float* __restrict__ rho_w, float2* __restrict__ pw,
+    float* __restrict__ flux, uint2* __restrict__ nbw, float4* __restrict__ nrec,
+    float* __restrict__ rho_raw) {
+    const DevParams& P = *Pd;
+    const int sub = threadIdx.x & (kDensitySplit - 1);
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kDensitySplit);
+    if (i >= n) return;  // whole groups: the lanes of a group share i
+    const float4 pi = pos[i];
+    const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
+    const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
+    const float T = P.r2_lt;
+    float acc = 0.0f;
+    uint2 e0 = make_uint2(0u, 0u);  // column 0's entry, written last
+    bool tail = false;
+#pragma unroll 1
+    for (int c = 0; c < 9; ++c) {
+        int b, e;
+        column_range(P, offset, pi.x, pi.y, pi.z, ix, iz, c, b, e);
+        const float4* __restrict__ q = pos + b;
+        const int len = e - b;
+        uint32_t lo = 0, hi = 0, none = 0;
+        int k = sub;
+        for (; k < len && k < 32; k += kDensitySplit) density_candidate(pi, q[k], T, acc, lo, k);
+        for (; k < len && k < 64; k += kDensitySplit)
+            density_candidate(pi, q[k], T, acc, hi, k - 32);
+        for (; k < len; k += kDensitySplit) density_candidate(pi, q[k], T, acc, none, 0);
+#pragma unroll
+        for (int o = 1; o < kDensitySplit; o <<= 1) {
+            lo |= __shfl_xor(lo, o, 64);
+            hi |= __shfl_xor(hi, o, 64);
         }
+        tail |= len > 64;
+        const uint32_t self = (uint32_t)(i - b);  // i is a candidate of one column
+        if (self < 32u) lo &= ~(1u << self);
+        else if (self < 64u) hi &= ~(1u << (self - 32u));
+        const uint2 ent = make_uint2(lo, (uint32_t)b | (hi ? kNbHi : 0u));
+        if (c == 0) e0 = ent;
+        else if (sub == 0) nbw[(size_t)c * n + i] = ent;
+        if (hi && sub == 0) nbw[(size_t)(9 + c) * n + i] = make_uint2(hi, (uint32_t)b + 32u);
     }
-    flux[i] = fl;
+#pragma unroll
+    for (int o = 1; o < kDensitySplit; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    if (sub != 0) return;
+    nbw[i] = make_uint2(e0.x, e0.y | (tail ? kNbTail : 0u));
+    density_finish(P, i, pi, vel[i], acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
 // Pressure + viscosity contribution of one neighbour (force_comp.glsl:85-86).
@@ -864,8 +933,12 @@ hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int
                           const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbw,
                           float4* nrec, float* rho_raw, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, B.pos, B.vel, offset, n, H0, rho_w, pw,
-                                                  flux, nbw, nrec, rho_raw);
+    if (n <= kDensitySplitMaxN)
+        k_density_split<<<blocks_for((long long)n * kDensitySplit, 256), 256, 0, st>>>(
+            P.dev, B.pos, B.vel, offset, n, H0, rho_w, pw, flux, nbw, nrec, rho_raw);
+    else
+        k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, B.pos, B.vel, offset, n, H0, rho_w,
+                                                      pw, flux, nbw, nrec, rho_raw);
     return hipGetLastError();
 }
 
