@@ -66,6 +66,7 @@ class Tape {
         const float* h1 = nullptr;
     };
     void alloc_step_tape(StepTape& t);
+    void release_pipeline();
     void save(Ckpt& c);
     void restore(const Ckpt& c, bool heights = true);
     void record_step(std::vector<StepTape>& set, int k, const Ckpt& c);
@@ -126,9 +127,14 @@ Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
             tp2_.resize(tp_.size());
             for (auto& t : tp2_) alloc_step_tape(t);
         }
-        hip_check(hipStreamCreateWithFlags(&ast_, hipStreamNonBlocking), "adjoint stream");
-        for (hipEvent_t& e : ev_)
-            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        try {
+            hip_check(hipStreamCreateWithFlags(&ast_, hipStreamNonBlocking), "adjoint stream");
+            for (hipEvent_t& e : ev_)
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        } catch (...) {
+            release_pipeline();
+            throw;
+        }
     }
     for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
     adjrec_.resize(2 * n);
@@ -141,13 +147,19 @@ Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
     acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
 }
 
-Tape::~Tape() {
+Tape::~Tape() { release_pipeline(); }
+
+void Tape::release_pipeline() {
     if (ast_) {
         (void)hipStreamSynchronize(ast_);
         (void)hipStreamDestroy(ast_);
+        ast_ = nullptr;
     }
-    for (hipEvent_t e : ev_)
-        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : ev_)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
 }
 
 void Tape::alloc_step_tape(StepTape& t) {
