@@ -283,7 +283,10 @@ typedef struct cwa_grad {
 typedef struct cwa_tape cwa_tape;
 
 /* A tape over `nsteps` steps of scene s from its current state, with a
- * checkpoint every `ckpt_every` steps.  s must outlive the tape. */
+ * checkpoint every `ckpt_every` steps.  s must outlive the tape.  Device
+ * memory: nsteps / ckpt_every + 1 checkpoints of the particle state and two
+ * heightfields, and two step tapes of ckpt_every steps each (one recorded
+ * while the adjoint walks the other). */
 int cwa_tape_create(cwa_scene* s, int nsteps, int ckpt_every, cwa_tape** out);
 /* Run the nsteps (the scene ends in the final state), storing checkpoints. */
 int cwa_tape_forward(cwa_tape* t, float dt);
