@@ -213,6 +213,46 @@ __device__ __forceinline__ bool coupled_(const DevParams& P, float y, float H) {
 // and use(record) consumes it.  U records are loaded before the first of
 // them is used; records are used in order.  st: the thread's LDS staging
 // slots (kNbStage of them, stride 256).
+// for_each_neighbour for lane sub of a group of G lanes sharing particle
+// i: the lane visits the neighbours whose position in the walk order is
+// sub mod G (G = 1: all of them).
+template <int U, int G, class Load, class Use>
+__device__ __forceinline__ void for_each_neighbour_of_lane(
+    const DevParams& P, const int* __restrict__ offset, const uint2* __restrict__ nbw,
+    const float4* __restrict__ pos, int n, int i, float4 pi, uint2* st, int sub, Load&& load,
+    Use&& use) {
+    const int nw = stage_nb_words(nbw, n, i, st, 256);
+    if (nw < 0) {
+        int t = 0;
+        for_each_candidate_neighbour(P, offset, pos, i, pi, [&](int j) {
+            if ((t++ % G) == sub) use(load(j));
+        });
+        return;
+    }
+    NbCursor nb(st, 256, nw);
+    for (int s = 0; s < sub; ++s) (void)nb.next();
+    const auto next = [&]() {
+        const int j = nb.next();
+#pragma unroll
+        for (int s = 1; s < G; ++s) (void)nb.next();
+        return j;
+    };
+    for (;;) {
+        int j[U];
+        j[0] = next();
+        if (j[0] < 0) break;
+#pragma unroll
+        for (int u = 1; u < U; ++u) j[u] = next();
+        decltype(load(0)) r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) r[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) use(r[u]);
+    }
+}
+
 template <int U, class Load, class Use>
 __device__ __forceinline__ void for_each_neighbour(const DevParams& P,
                                                    const int* __restrict__ offset,
@@ -512,7 +552,22 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
 // ---------------------------------------------------------------------------
 // Pair adjoint of the pressure and viscosity sums, then the density-side
 // adjoint of particle k (EOS, rest-density clamp, wave term, flux).
+// G lanes per particle (consecutive lanes of a wave): each walks every G-th
+// neighbour, the group's sums are added in a fixed tree and its first lane
+// finishes the particle.  G = 4 for scenes of up to kAdjSplitMaxN
+// particles, which would otherwise keep one wave per SIMD.
 // ---------------------------------------------------------------------------
+constexpr int kAdjSplit = 4;
+constexpr int kAdjSplitMaxN = 1 << 18;
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int G>
 __global__ void __launch_bounds__(256) k_adj_pairs(
     const DevParams* __restrict__ Pd, float dt, int n, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const int* __restrict__ offset,
@@ -524,23 +579,24 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
     float4* __restrict__ vb_loc, float* __restrict__ rtb, double* __restrict__ acc) {
     const DevParams& P = *Pd;
     __shared__ uint2 s_nb[kNbStage][256];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const int sub = (int)threadIdx.x % G;
     float part[3] = {0.f, 0.f, 0.f};  // visc, gas_const, rest_rho
-    if (k < n) {
+    float pkb = 0.f, irkb = 0.f;
+    float xbx = 0.f, xby = 0.f, xbz = 0.f, vbx = 0.f, vby = 0.f, vbz = 0.f;
+    float Vx = 0.f, Vy = 0.f, Vz = 0.f;  // viscosity sum of k (for d/d visc)
+    if (k < n) {  // whole groups: the lanes of a group share k
         const float4 xk = pos[k], vk = vel[k];
         const float2 qk = pw[k];
         const float pk = qk.x, irk = qk.y;
         const float4 Fb = adjrec[2 * (size_t)k], Vb = adjrec[2 * (size_t)k + 1];
         const float msh = P.m * P.spiky * 0.5f, mlap = P.m * P.lap;
-        float pkb = 0.f, irkb = 0.f;
-        float xbx = 0.f, xby = 0.f, xbz = 0.f, vbx = 0.f, vby = 0.f, vbz = 0.f;
-        float Vx = 0.f, Vy = 0.f, Vz = 0.f;  // viscosity sum of k (for d/d visc)
         const auto load = [&](int j) {
             return AdjNbr{nrec[2 * (size_t)j], nrec[2 * (size_t)j + 1], adjrec[2 * (size_t)j],
                           adjrec[2 * (size_t)j + 1]};
         };
-        for_each_neighbour<2>(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x], load,
-                              [&](const AdjNbr& q) {
+        for_each_neighbour_of_lane<2, G>(P, offset, nbw, pos, n, k, xk, &s_nb[0][threadIdx.x],
+                                         sub, load, [&](const AdjNbr& q) {
             const float4 ra = q.ra, rb = q.rb, Fbj = q.Fbj, Vbj = q.Vbj;
             const float pj = ra.w, irj = rb.w;
             const float dx = xk.x - ra.x, dy = xk.y - ra.y, dz = xk.z - ra.z;
@@ -595,6 +651,24 @@ __global__ void __launch_bounds__(256) k_adj_pairs(
             xby += gy;
             xbz += gz;
         });
+        if (G > 1) {
+            pkb = group_sum<G>(pkb);
+            irkb = group_sum<G>(irkb);
+            xbx = group_sum<G>(xbx);
+            xby = group_sum<G>(xby);
+            xbz = group_sum<G>(xbz);
+            vbx = group_sum<G>(vbx);
+            vby = group_sum<G>(vby);
+            vbz = group_sum<G>(vbz);
+            Vx = group_sum<G>(Vx);
+            Vy = group_sum<G>(Vy);
+            Vz = group_sum<G>(Vz);
+        }
+    }
+    if (k < n && sub == 0) {
+        const float4 xk = pos[k], vk = vel[k];
+        const float irk = pw[k].y;
+        const float4 Fb = adjrec[2 * (size_t)k], Vb = adjrec[2 * (size_t)k + 1];
         part[0] = Vb.w * (Fb.x * Vx + Fb.y * Vy + Fb.z * Vz);
         // ---- density side ----
         const float rhob = rhob_d[k] - irkb * irk * irk;
@@ -689,7 +763,8 @@ __global__ void __launch_bounds__(256) k_adj_rho_pairs(
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
 int adj_wave_blocks(const DevParams& P) { return ((P.W + 63) / 64) * ((P.H + 3) / 4); }
-int adj_pair_blocks(int n) { return nblk(n, 256); }
+static int adj_pair_group(int n) { return n <= kAdjSplitMaxN ? kAdjSplit : 1; }
+int adj_pair_blocks(int n) { return nblk((long long)n * adj_pair_group(n), 256); }
 
 // Loss seed of a heightfield fit: L = mean((h - t)^2), dL/dh = 2 (h - t) / N.
 // Per-block partial sums (double, fixed order: grid-stride per thread, then
@@ -772,9 +847,14 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_adj_pairs<<<nblk(n, 256), 256, 0, st>>>(P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w,
-                                              pw, rho_raw, H0, Sbar, adjrec, rhob_d, G0n,
-                                              xb_loc, vb_loc, rtb, acc);
+    if (adj_pair_group(n) == kAdjSplit)
+        k_adj_pairs<kAdjSplit><<<adj_pair_blocks(n), 256, 0, st>>>(
+            P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w, pw, rho_raw, H0, Sbar, adjrec,
+            rhob_d, G0n, xb_loc, vb_loc, rtb, acc);
+    else
+        k_adj_pairs<1><<<adj_pair_blocks(n), 256, 0, st>>>(
+            P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w, pw, rho_raw, H0, Sbar, adjrec,
+            rhob_d, G0n, xb_loc, vb_loc, rtb, acc);
     return hipGetLastError();
 }
 
