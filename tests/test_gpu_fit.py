@@ -115,3 +115,71 @@ def test_adam_descent_lowers_every_loss(gpu, cwa):
     assert np.all(np.isfinite(res.losses))
     assert np.all(res.losses[-1] < res.losses[0])
     assert np.all(res.tape_bytes > 0) and np.all(res.scene_bytes > 0)
+
+
+def test_fit_workload_64_scenes(gpu, cwa, oracle):
+    """The batched-fit workload itself: 64 scenes of 8k-128k particles on
+    128^2-1024^2 heightfields, 128 steps checkpointed every 16, all in one
+    graph.  For every scene the loss and the gradients are finite and the
+    graph's loss equals the host MSE of the scene's final heightfield
+    (1e-9 rel); for the smallest and the largest scene the gradients equal a
+    single eager tape's (1e-4 rel); the smallest scene's 128-step forward is
+    within 3x the envelope of FP32 oracle runs (the oracle and four runs from
+    positions moved one ulp) of the FP64 oracle, per field."""
+    from coupledwateranimation_amd import fit
+    from coupledwateranimation_amd.fit import FitBatch
+    from coupledwateranimation_amd.scene import Scene
+
+    specs, targets, _ = fit.fit_workload(nscenes=64)
+    ns = np.array([sp.n for sp in specs])
+    _say(f"fit workload: {len(specs)} scenes, {ns.sum()} particles, steps {specs[0].steps}")
+    scenes = [Scene(sp.params, sp.pos, sp.vel, None, sp.wave_size, sp.height0, sp.height1)
+              for sp in specs]
+    fb = FitBatch(scenes, targets, specs[0].steps, 16, specs[0].dt)
+    losses, grads = fb.run()
+    assert np.all(np.isfinite(losses)) and np.all(losses > 0)
+    for s, sc in enumerate(scenes):
+        assert all(np.isfinite(grads[s][k]) for k in SCALARS), s
+        h = sc.readback()["height0"].astype(np.float64)
+        mse = float(((h - targets[s].astype(np.float64)) ** 2).mean())
+        assert _rel(losses[s], mse) < 1e-9, (s, losses[s], mse)
+    small, large = int(np.argmin(ns)), int(np.argmax(ns))
+    for s in (small, large):
+        sp = specs[s]
+        loss_e, g_e = _eager(sp, sp.params, targets[s], sp.steps, 16)
+        _say(f"scene {s}: n={sp.n} wave={sp.wave_size} loss graph {losses[s]:.6e} "
+             f"eager {loss_e:.6e}")
+        assert _rel(losses[s], loss_e) < 1e-9
+        for k in SCALARS:
+            assert _rel(grads[s][k], g_e[k]) < 1e-4, (s, k, grads[s][k], g_e[k])
+    st = scenes[small].readback()
+    fb.close()
+    for sc in scenes:
+        sc.close()
+    sp = specs[small]
+    ref64 = oracle.OracleScene(sp.params, sp.pos, sp.vel, None, sp.wave_size, sp.height0,
+                               sp.height1, double=True, threads=16)
+    ref64.step(sp.dt, sp.steps)
+    f64 = ref64.state()
+    env = {}
+    for k in range(5):
+        pos = sp.pos.copy()
+        if k:
+            m = np.random.default_rng(k).random(pos[:, :3].shape) < 0.5
+            pos[:, :3] = np.where(m, np.nextafter(pos[:, :3], np.float32(2)), pos[:, :3])
+        o = oracle.OracleScene(sp.params, pos, sp.vel, None, sp.wave_size, sp.height0,
+                               sp.height1, threads=16)
+        o.step(sp.dt, sp.steps)
+        f32 = o.state()
+        o.close()
+        for key, sl in (("pos", np.s_[:, :3]), ("vel", np.s_[:, :3]), ("rho", np.s_[:]),
+                        ("height0", np.s_[:])):
+            e = np.abs(f32[key][sl] - f64[key][sl]).max() / max(np.abs(f64[key][sl]).max(),
+                                                                1e-30)
+            env[key] = max(env.get(key, 0.0), e)
+    ref64.close()
+    for key, sl in (("pos", np.s_[:, :3]), ("vel", np.s_[:, :3]), ("rho", np.s_[:]),
+                    ("height0", np.s_[:])):
+        e = np.abs(st[key][sl] - f64[key][sl]).max() / max(np.abs(f64[key][sl]).max(), 1e-30)
+        _say(f"smallest scene {key}: gpu-vs-fp64 {e:.3e}, fp32 envelope {env[key]:.3e}")
+        assert e <= 3 * env[key] + 1e-6, key

@@ -256,23 +256,68 @@ def test_one_step_parity(gpu, cwa, oracle, name):
                        atol=1e-6 * max(np.abs(og["colflux"]).max(), 1e-30))
 
 
+_FIELDS = (("pos", np.s_[:, :3]), ("vel", np.s_[:, :3]), ("rho", np.s_[:]), ("prs", np.s_[:]),
+           ("height0", np.s_[:]))
+
+
+def _ulp_nudged(pos, seed):
+    """pos with a random half of its coordinates moved one ulp up: the same
+    scene with a different fp32 rounding history."""
+    rng = np.random.default_rng(seed)
+    out = pos.copy()
+    m = rng.random(out[:, :3].shape) < 0.5
+    out[:, :3] = np.where(m, np.nextafter(out[:, :3], np.float32(2)), out[:, :3])
+    return out
+
+
 @pytest.mark.parametrize("name", ["dam_break_4k", "dam_break_4k_ydiv4"])
 def test_n_step_parity_vs_fp64(gpu, cwa, oracle, name):
-    """50 coupled steps of the 4 096-particle dam break (y rows of h and of
-    h/4).  Error of the GPU against the FP64 oracle, per field, must stay
-    within 20x the FP32 oracle's own error plus a floor of 1e-6 (relative
-    to the field max)."""
+    """The full 200 steps of the 4 096-particle + 128^2 dam break (y rows of
+    h and of h/4), error of each field against the FP64 oracle (max over
+    particles / texels, relative to the field max) at steps 50, 100 and 200.
+
+    Steps 50 and 100: the GPU's error stays within 3x the FP32 oracle's own
+    error plus a floor of 1e-6.  By step 200 the dam break is chaotic: fp32
+    runs that differ only in rounding (the oracle from positions moved one
+    ulp in a random half of their coordinates) spread up to 5x apart in max
+    error, so the bar there is 3x the envelope (largest error) of the FP32
+    oracle and four such nudged FP32 runs, for the max and for the rms
+    error."""
     spec = _specs(cwa)[name]
-    steps = 50
-    st, _ = _gpu_run(cwa, spec, steps)
-    _, f32 = _oracle_run(oracle, spec, steps)
-    _, f64 = _oracle_run(oracle, spec, steps, double=True)
-    for k, sl in (("pos", np.s_[:, :3]), ("vel", np.s_[:, :3]), ("rho", np.s_[:]),
-                  ("prs", np.s_[:]), ("height0", np.s_[:])):
-        e_gpu = _rel(st[k][sl], f64[k][sl])
-        e_f32 = _rel(f32[k][sl], f64[k][sl])
-        _say(f"{k}: gpu-vs-fp64 {e_gpu:.3e}  fp32oracle-vs-fp64 {e_f32:.3e}")
-        assert e_gpu <= 20 * e_f32 + 1e-6, k
+    marks = (50, 100, 200)
+    s = spec.make_scene()
+    ref64 = oracle.OracleScene(spec.params, spec.pos, spec.vel, None, spec.wave_size,
+                               spec.height0, spec.height1, double=True, threads=16)
+    ens = [oracle.OracleScene(spec.params, spec.pos if k == 0 else _ulp_nudged(spec.pos, k),
+                              spec.vel, None, spec.wave_size, spec.height0, spec.height1,
+                              threads=16) for k in range(5)]
+    done = 0
+    for m in marks:
+        s.step(spec.dt, m - done)
+        for o in [ref64] + ens:
+            o.step(spec.dt, m - done)
+        done = m
+        st, f64 = s.readback(), ref64.state()
+        states = [o.state() for o in (ens if m > 100 else ens[:1])]
+        for k, sl in _FIELDS:
+            scale = max(np.abs(f64[k][sl]).max(), 1e-30)
+
+            def errs(a):
+                d = np.abs(a[k][sl].astype(np.float64) - f64[k][sl])
+                d = d.reshape(d.shape[0], -1).max(1)
+                return d.max() / scale, np.sqrt((d ** 2).mean()) / scale
+
+            e_gpu, r_gpu = errs(st)
+            env = [errs(x) for x in states]
+            e_env, r_env = max(e for e, _ in env), max(r for _, r in env)
+            _say(f"step {m} {k}: gpu-vs-fp64 max {e_gpu:.3e} rms {r_gpu:.3e}; fp32 "
+                 f"{'oracle' if m <= 100 else 'envelope'} max {e_env:.3e} rms {r_env:.3e}")
+            assert e_gpu <= 3 * e_env + 1e-6, (m, k)
+            if m > 100:
+                assert r_gpu <= 3 * r_env + 1e-7, (m, k)
+    s.close()
+    for o in [ref64] + ens:
+        o.close()
 
 
 def test_branch_scenes(gpu, cwa, oracle):
@@ -445,9 +490,13 @@ def test_empty_and_edge_scenes(gpu, cwa):
 def test_large_scene_step(gpu, cwa, oracle):
     """1 048 576 particles + 2048^2 heightfield: one step against the oracle
     (16 threads; grid arrays and every particle's neighbour count bit-exact),
-    then 10 more steps checked through size-independent
-    properties: ids stay a permutation, slots stay cell-sorted, all finite,
-    particles inside the box."""
+    then the rest of the workload's 1 000 steps (hipGraph replays) checked
+    through size-independent properties: ids stay a permutation, slots stay
+    cell-sorted, all finite, particles inside the box; the last step's grid
+    arrays equal the oracle's grid build on the slot-ordered positions it
+    started from, and every particle's neighbour count in that step's
+    density pass equals the oracle's grid search on those positions, after
+    the dam has collapsed."""
     from coupledwateranimation_amd import configs
 
     spec = configs.dam_break_large(steps=1)
@@ -467,16 +516,35 @@ def test_large_scene_step(gpu, cwa, oracle):
     assert _rel(st["force"][:, :3], ref["force"][:, :3]) < 2e-5
     assert np.abs(st["pos"][:, :3] - ref["pos"][:, :3]).max() < 1e-6
     assert _rel(st["height0"], ref["height0"]) < 1e-6
-    _say("large: 10 more steps")
-    s.step(spec.dt, 10)
-    ids = s.slot_ids()
-    assert np.array_equal(np.sort(ids), np.arange(spec.n))
-    st = s.readback()
-    g = s.grid_host()
-    assert np.isfinite(st["pos"]).all() and np.isfinite(st["height0"]).all()
-    assert np.all(np.diff(g["key"][g["list"]]) >= 0)
     lo, hi = np.array(spec.params.lower), np.array(spec.params.upper)
-    assert np.all(st["pos"][:, :3] >= lo) and np.all(st["pos"][:, :3] <= hi)
+    done = 1
+    for upto in (11, 999):
+        _say(f"large: steps {done}..{upto}")
+        s.step(spec.dt, upto - done, graph=True)
+        done = upto
+        ids = s.slot_ids()
+        assert np.array_equal(np.sort(ids), np.arange(spec.n))
+        st = s.readback()
+        g = s.grid_host()
+        assert np.isfinite(st["pos"]).all() and np.isfinite(st["height0"]).all()
+        assert np.all(np.diff(g["key"][g["list"]]) >= 0)
+        assert np.all(st["pos"][:, :3] >= lo) and np.all(st["pos"][:, :3] <= hi)
+    _say("large: step 1000, grid and neighbour counts against the oracle")
+    pos_slots = st["pos"][ids]
+    s.step(spec.dt, 1)
+    g = s.grid_host()
+    r = oracle.grid_build(spec.params, pos_slots)
+    for k in ("key", "count", "offset", "list"):
+        assert np.array_equal(g[k], r[k]), k
+    nb = s.neighbour_counts()
+    ref_nb = oracle.neighbor_counts(spec.params, st["pos"], all_pairs=False)
+    assert np.array_equal(nb, ref_nb), f"{np.count_nonzero(nb != ref_nb)} particles differ"
+    moved = np.abs(st["pos"][:, :3] - spec.pos[:, :3]).max()
+    _say(f"large: after 999 steps max displacement {moved:.4f}, neighbours mean "
+         f"{nb.mean():.1f} max {nb.max()}")
+    st = s.readback()
+    assert np.isfinite(st["pos"]).all() and np.all(st["pos"][:, :3] >= lo) and \
+        np.all(st["pos"][:, :3] <= hi)
     s.close()
 
 
