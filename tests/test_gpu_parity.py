@@ -322,7 +322,10 @@ def test_n_step_parity_vs_fp64(gpu, cwa, oracle, name):
 
 def test_branch_scenes(gpu, cwa, oracle):
     """Tiny scenes that drive every branch: boundary faces, foam, surface
-    clamp, coupling band with flux, wake ceiling mode, coincident pair."""
+    clamp, coupling band with flux, wake mode with the reference's ceiling
+    (+0.1, never reached) and with one below the box top (-0.2: a particle
+    rising through it and one leaving through the top, reflected twice),
+    coincident pair."""
     from coupledwateranimation_amd import configs
 
     base = configs.dam_break_small()
@@ -335,18 +338,20 @@ def test_branch_scenes(gpu, cwa, oracle):
                     [0.5, 0.9995, 0.5, 1], [0.5, 0.5, 0.0005, 1], [0.5, 0.5, 0.9995, 1],
                     [0.3, 0.4, 0.3, 1], [0.7, 0.03, 0.2, 1], [0.2, 0.01, 0.8, 1],
                     [0.0, 0.0, 0.0, 1], [0.0, 0.0, 0.0, 1], [0.6, 0.035, 0.6, 1],
-                    [0.61, 0.04, 0.6, 1]], np.float32)
+                    [0.61, 0.04, 0.6, 1], [0.4, 0.85, 0.7, 1]], np.float32)
     vel = np.zeros_like(pos)
     vel[:6, :3] = [[-2, 0, 0], [2, 0, 0], [0, -2, 0], [0, 2, 0], [0, 0, -2], [0, 0, 2]]
     vel[6] = [0, 0, 30, 0]          # foam
     vel[7] = [0.1, -0.5, 0, 0]      # in band, flux
     vel[11] = [0, -0.3, 0.1, 0]
+    vel[13] = [0, 2, 0, 0]          # rises through a ceiling at 0.8
     force = np.zeros_like(pos)
     force[:, :3] = [1.0, -2.0, 0.5]
     force[6, 3] = 2.0
-    for wtype in (1.0, 0.5):
+    for wtype, extra in ((1.0, 0.1), (0.5, 0.1), (0.5, -0.2)):
         q = p.copy()
         q.wave_type = wtype
+        q.ceiling_extra = extra
         s = cwa.Scene(q, pos, vel, force, (W, H), h0, h0)
         s.step(1e-3, 3)
         st = s.readback()

@@ -18,7 +18,13 @@ index direction, a dropped term) fails it:
     particles repel with equal and opposite accelerations whose magnitude
     follows from the pinned kernel;
   * the particle -> wave source conserves the column flux;
-  * the enable flag keeps disabled particles in place.
+  * the enable flag keeps disabled particles in place;
+  * the viscosity term is mu lap v (the 2(d+2) SPH Laplacian) on a
+    quadratic shear, Galilean invariant, radial only and dissipative;
+  * the boundary density term's closed forms (2 rho0/3 on a plane, rho0/6
+    at H/2) at all four planes, and the boundary pressure force is the
+    pressure-weighted gradient of that density term (central differences);
+  * the wave pressure's ix % 5 gate and both smoothstep ramps.
 Trajectories of the reference itself: parity unpinned (the GLSL cannot be
 run here); the FP64 build is the error reference for the float build.
 """
@@ -380,3 +386,213 @@ def test_shallow_reference_scene_runs_and_couples(o2):
     st = o.state()
     assert np.isfinite(st["xv"]).all() and np.isfinite(st["h0"]).all()
     assert np.abs(st["h0"] - spec.h0).max() > 1e-3
+
+
+# ------------------------------------------------- viscosity, boundary, wave pressure
+def _spec1(p, xv, mass, h0=None, h1=None, Lw=16, dt=1e-6):
+    xv = np.asarray(xv, np.float32).reshape(-1, 4)
+    z = np.zeros(Lw, np.float32)
+    return batch2d.BatchSpec(params=p, n=np.array([xv.shape[0]]), wave_len=np.array([Lw]),
+                             wave_substeps=np.array([1]), mass=np.array([mass], np.float32),
+                             lam=np.array([0.0], np.float32), xv=xv,
+                             h0=z if h0 is None else np.asarray(h0, np.float32),
+                             h1=z if h1 is None else np.asarray(h1, np.float32), dt=dt, steps=1)
+
+
+def _accel(o2, spec):
+    """(x, y, ax, ay) per particle (original order) after one double step."""
+    o = o2.OracleBatch(spec, double=True)
+    o.step(spec.dt, 1)
+    st = o.state()
+    o.close()
+    a = (st["xv"][:, 2:4] - spec.xv[:, 2:4].astype(np.float64)) / spec.dt
+    return st, a
+
+
+def test_viscosity_is_nu_laplacian(o2):
+    """The viscosity term (SphWaveKoschier2D_grid_cs.glsl:504,541,550:
+    -8 mu m sum (1/rho_j) (v_ij . r_ij) / (r^2 + 0.01 H^2) W'(r) u_ij) is
+    the 2(d + 2) form of the SPH Laplacian, mu lap v.  For the shear
+    v = (a (y - y0)^2, 0), lap v = (2a, 0) and div v = 0, so in the
+    continuum limit the acceleration is (2 a mu C, 0) with
+    C = int r^4/(r^2 + 0.01 H^2) W' dr / int r^2 W' dr = 0.95068 the effect
+    of the regularisation (quadrature below).  A lattice at H/s = 8 with
+    k = 0 (no pressure) and no gravity reaches it within 0.1 %.  A slip in
+    the 8, the 0.01, the r^2, the dot product or the sign fails; so does a
+    velocity term that is not Galilean invariant (a uniform drift is added
+    and must change nothing)."""
+    from scipy.integrate import quad
+
+    H = 0.04
+    Hf = float(np.float32(H))
+    num = quad(lambda r: r ** 4 / (r * r + 0.01 * Hf * Hf) * o2.Wgrad(_params(h=H), r), 0, Hf,
+               points=[0.5 * Hf], epsabs=0, epsrel=1e-12)[0]
+    den = quad(lambda r: r * r * o2.Wgrad(_params(h=H), r), 0, Hf, points=[0.5 * Hf],
+               epsabs=0, epsrel=1e-12)[0]
+    creg = num / den
+    assert den == pytest.approx(-1.0 / math.pi, rel=1e-6)   # int W dA = 1, by parts
+    a, mu = 3.0, 0.01
+    p = _params(h=H, gas_const=0.0, visc=mu, wave_press=0.0, couple_band=0.0,
+                couple_strength=0.0)
+    p.gravity[1] = 0.0
+    s = float(np.float32(H / 8))
+    n1 = int(0.5 / s)
+    i, j = np.meshgrid(np.arange(n1), np.arange(n1), indexing="ij")
+    out = []
+    for drift in ((0.0, 0.0), (0.3, -0.2)):
+        xv = np.zeros((n1 * n1, 4), np.float32)
+        xv[:, 0] = 0.25 + i.ravel() * s
+        xv[:, 1] = 0.25 + j.ravel() * s
+        xv[:, 2] = a * (xv[:, 1].astype(np.float64) - 0.5) ** 2 + drift[0]
+        xv[:, 3] = drift[1]
+        spec = _spec1(p, xv, 1000.0 * s * s)
+        _, acc = _accel(o2, spec)
+        inner = (np.abs(xv[:, 0] - 0.5) < 0.08) & (np.abs(xv[:, 1] - 0.5) < 0.08)
+        out.append(acc[inner])
+    ref = 2.0 * a * mu * creg
+    np.testing.assert_allclose(out[0][:, 0], ref, rtol=1e-3)
+    assert np.abs(out[0][:, 1]).max() < 1e-3 * ref
+    np.testing.assert_allclose(out[1], out[0], rtol=0, atol=2e-4 * ref)
+
+
+def test_viscosity_pair_radial_only_and_dissipative(o2):
+    """Two particles: a purely tangential relative velocity gives no viscous
+    force (the term carries v_ij . r_ij); an approaching pair is slowed with
+    equal and opposite accelerations along the line of centres (momentum
+    conserved, relative kinetic energy dissipated)."""
+    H = 0.05
+    p = _params(h=H, gas_const=0.0, visc=0.5, wave_press=0.0, couple_band=0.0,
+                couple_strength=0.0)
+    p.gravity[1] = 0.0
+    r = 0.4 * H
+    tang = [[0.5, 0.5, 0.0, 1.0], [0.5 + r, 0.5, 0.0, -1.0]]
+    _, acc = _accel(o2, _spec1(p, tang, 1.0))
+    assert np.all(acc == 0.0)
+    appr = [[0.5, 0.5, 1.0, 0.0], [0.5 + r, 0.5, -1.0, 0.0]]
+    _, acc = _accel(o2, _spec1(p, appr, 1.0))
+    assert acc[0, 0] < 0.0 < acc[1, 0]               # both slowed
+    assert acc[0, 0] == pytest.approx(-acc[1, 0], rel=1e-12)
+    assert acc[0, 1] == 0.0 and acc[1, 1] == 0.0
+
+
+def _wall_points(p, d):
+    """(x, y, inward normal) at distance d from each of the four planes,
+    midway along it, far from the other three."""
+    w = [float(np.float32(v)) for v in p.walls]
+    return [((0.5, w[0] + d), (0.0, 1.0)), ((w[1] + d, 0.5), (1.0, 0.0)),
+            ((w[2] - d, 0.5), (-1.0, 0.0)), ((0.5, w[3] - d), (0.0, -1.0))]
+
+
+def test_boundary_density_term(o2):
+    """rho += PSI W(max(0, d)) for d < H (SphWaveKoschier2D_grid_cs.glsl:
+    474-477), PSI = rho0 / (1.5 k2): with W(0) = k2 and W(H/2) = k2/4 a
+    particle on a plane gains 2 rho0 / 3 and one at H/2 gains rho0 / 6,
+    nothing from H on, the same at each of the four planes and anywhere
+    along a plane.  A particle pushed to depth 0.2 H through a plane is
+    first moved back to 1/4 of its depth (:417-423), still behind it, so it
+    gains 2 rho0 / 3, while its own term is now taken between its moved and
+    its stored position, 0.15 H apart: m W(0.15 H) = rho0 (1 - 6 q^2 (1 - q))
+    at q = 0.15.  The mass is set so that the particle's own term is rho0
+    (m W(0))."""
+    H = 0.05
+    p = _params(h=H, gas_const=0.0, visc=0.0, wave_press=0.0, couple_band=0.0,
+                couple_strength=0.0)
+    p.gravity[1] = 0.0
+    m = 1000.0 / o2.W(p, 0.0)
+    Hf = float(np.float32(H))
+    q = 0.15
+    for d, gain in ((0.0, 2.0 / 3.0), (0.5 * Hf, 1.0 / 6.0), (Hf, 0.0), (1.3 * Hf, 0.0),
+                    (-0.2 * Hf, 2.0 / 3.0 - 6 * q * q * (1 - q))):
+        for (x, y), _ in _wall_points(p, d):
+            for shift in (0.0, 0.17):
+                xs = x + shift if y != 0.5 else x
+                ys = y + shift if y == 0.5 else y
+                st, _ = _accel(o2, _spec1(p, [[xs, ys, 0.0, 0.0]], np.float32(m)))
+                rho = st["rp"][0, 0]
+                expect = 1000.0 * float(np.float32(m)) / m * (1.0 + gain)
+                assert rho == pytest.approx(expect, rel=2e-6), (d, x, y, shift)
+
+
+@pytest.mark.parametrize("frac", [0.2, 0.5, 0.8])
+def test_boundary_pressure_is_the_density_gradient(o2, frac):
+    """The boundary pressure force (:553-562) is the pressure term of the
+    boundary density (:474-477): a = -(p_i / rho_i^2) grad(PSI W(d)), with
+    the gradient taken by central differences of the pass's own density as
+    the particle moves along the normal.  Pins the sign (repulsive), the
+    normal of each plane, PSI and W' of the wall distance; at d = H/2 the
+    magnitude is also (p_i / rho_i^2) rho0 / H (W'(H/2) = -3 k2 / (2 H))."""
+    H = 0.05
+    p = _params(h=H, gas_const=30000.0, visc=0.0, wave_press=0.0, couple_band=0.0,
+                couple_strength=0.0)
+    p.gravity[1] = 0.0
+    m = np.float32(1000.0 / o2.W(p, 0.0))
+    Hf = float(np.float32(H))
+    d = frac * Hf
+    eps = 1e-5 * Hf
+    for (x, y), (nx, ny) in _wall_points(p, d):
+        st, acc = _accel(o2, _spec1(p, [[x, y, 0.0, 0.0]], m))
+        rho, prs = st["rp"][0]
+        grad, at = [], []
+        for sgn in (1.0, -1.0):
+            sp = _spec1(p, [[x + sgn * eps * nx, y + sgn * eps * ny, 0.0, 0.0]], m)
+            at.append(sp.xv[0, :2].astype(np.float64) @ np.array([nx, ny]))
+            grad.append(_accel(o2, sp)[0]["rp"][0, 0])
+        drho_dd = (grad[0] - grad[1]) / (at[0] - at[1])   # along the inward normal
+        expect = -(prs / rho ** 2) * drho_dd * np.array([nx, ny])
+        # (the kernel's second derivative jumps at H/2: the central difference
+        # straddling it is good to ~eps/H)
+        np.testing.assert_allclose(acc[0], expect, rtol=2e-5, atol=1e-9 * abs(expect).max())
+        assert acc[0] @ np.array([nx, ny]) > 0.0        # pushed away from the plane
+        if frac == 0.5:
+            # (PSI is rounded to fp32 and the particle sits at H/2 up to fp32)
+            assert np.linalg.norm(acc[0]) == pytest.approx(prs / rho ** 2 * 1000.0 / Hf,
+                                                           rel=1e-5)
+
+
+def test_wave_pressure_gate_and_edges(o2):
+    """acc.y -= mask * P * smoothstep(0, dy_max, dy) for particle index
+    ix % 5 < 4 (SphWaveKoschier2D_grid_cs.glsl:586-603), with
+    mask = smoothstep(0, v_max, |v_w|) and dy = y - h - (h - h_ref)/2.
+    Isolated particles over a flat wave moving at v_w = (h^n - h^(n-1))/dt:
+    particles 4, 9, ... feel nothing; the others feel -P where both
+    smoothsteps are saturated, about -P/2 near either midpoint, nothing at
+    or below dy = 0 or on a wave at rest; moving h away from h_ref moves the
+    ramp by (h - h_ref)/2.  Expected values use GLSL's smoothstep
+    (t = clamp(x / e, 0, 1), t^2 (3 - 2t)) on the fp32 inputs."""
+    H = 0.02
+    P = 20.0
+    p = _params(h=H, gas_const=0.0, visc=0.0, wave_press=P, couple_band=0.0,
+                couple_strength=0.0)
+    p.gravity[1] = 0.0
+    p.wave_ref = 0.3
+    p.wave_dy = 0.1
+    p.wave_maskv = 0.2
+    dt = 1e-3
+    href = float(np.float32(p.wave_ref))
+    dymax = float(np.float32(p.wave_dy))
+    vmax = float(np.float32(p.wave_maskv))
+
+    def ss(x, e):
+        t = min(max(x / e, 0.0), 1.0)
+        return t * t * (3.0 - 2.0 * t)
+
+    for hw, vw, dy, approx in ((0.3, 0.5, 0.2, -P), (0.3, 0.5, 0.05, -0.5 * P),
+                               (0.3, 0.1, 0.2, -0.5 * P), (0.3, 0.5, 0.0, 0.0),
+                               (0.3, 0.5, -0.05, 0.0), (0.3, 0.0, 0.2, 0.0),
+                               (0.4, -0.5, 0.05, -0.5 * P), (0.24, 0.5, 0.05, -0.5 * P),
+                               (0.3, 0.5, 0.08, None)):
+        h0v = np.float32(hw)
+        h1v = np.float32(h0v - np.float32(vw * dt))
+        y = float(h0v) + 0.5 * (float(h0v) - href) + dy
+        xs = 0.1 + 0.08 * np.arange(10)
+        xv = np.array([[x, y, 0.0, 0.0] for x in xs], np.float32)
+        st, acc = _accel(o2, _spec1(p, xv, 1e-4, h0=np.full(16, h0v), h1=np.full(16, h1v),
+                                    dt=dt))
+        gate = (np.arange(10) % 5) < 4
+        assert np.all(acc[~gate, 1] == 0.0) and np.all(acc[:, 0] == 0.0)
+        vwf = (float(h0v) - float(h1v)) / dt
+        dyf = (float(xv[0, 1]) - float(h0v)) - 0.5 * (float(h0v) - href)
+        expect = -P * ss(abs(vwf), vmax) * ss(dyf, dymax)
+        np.testing.assert_allclose(acc[gate, 1], expect, rtol=1e-9, atol=1e-12 * P)
+        if approx is not None:        # the ramps' saturation and midpoints
+            assert expect == pytest.approx(approx, rel=2e-3, abs=1e-12)

@@ -244,6 +244,32 @@ def test_foam_branch(oracle, cwa, speed, foam):
         assert st["force"][0, 3] == pytest.approx(2.0, rel=1e-12)
 
 
+@pytest.mark.parametrize("wtype,extra,y0,vy,y_end,vy_factor", [
+    (0.5, -0.2, 0.85, 2.0, 0.8, -0.3),     # wake mode: ceiling at upper.y - 0.2
+    (1.0, -0.2, 0.85, 2.0, None, 1.0),     # splash mode: no ceiling
+    (0.5, 0.1, 0.9995, 2.0, 1.0, -0.3),    # the reference's +0.1: box clamp only
+    (0.5, -0.2, 0.9995, 2.0, 0.8, 0.09),   # box clamp, then the ceiling: two bounces
+])
+def test_wake_ceiling(oracle, cwa, wtype, extra, y0, vy, y_end, vy_factor):
+    """CheckBoundary's wake-mode ceiling (integrate_comp.glsl:170-177): for
+    0 < wave type < 1, after the box clamps, y > upper.y + extra is clamped
+    to upper.y + extra with v.y *= -bounce.  With the reference's extra of
+    +0.1 it can never fire (the box already clamped y to upper.y); with a
+    negative extra it does, and a particle leaving through the box top is
+    reflected twice (v.y *= (-0.3)^2), in the shader's order."""
+    p = _free_params(cwa)
+    p.wave_type = wtype
+    p.ceiling_extra = extra
+    dt = 1e-3
+    st = _run(oracle, p, _pts((0.5, y0, 0.5)), vel=np.array([[0.0, vy, 0.0, 0.0]]), dt=dt)
+    if y_end is None:
+        assert st["pos"][0, 1] == pytest.approx(y0 + dt * vy, rel=1e-12)
+        assert st["vel"][0, 1] == pytest.approx(vy, rel=1e-12)
+    else:
+        assert st["pos"][0, 1] == pytest.approx(y_end, rel=1e-7)
+        assert st["vel"][0, 1] == pytest.approx(vy_factor * vy, rel=1e-7)
+
+
 def test_surface_clamp(oracle, cwa):
     """y < H -> y = H - particle_radius (integrate_comp.glsl:79-83)."""
     p = _free_params(cwa)
