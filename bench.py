@@ -68,7 +68,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="timed oracle steps of the CPU baseline, about 10 s on 16 host "
                          "cores (0 disables it)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: every core in the affinity mask, bounded by OMP_NUM_THREADS "
+                         "when set")
     ap.add_argument("--adjoint-steps", type=int, default=256,
                     help="steps of the forward+adjoint measurement (0 disables it)")
     ap.add_argument("--batched-steps", type=int, default=500,
@@ -269,14 +271,14 @@ def measure_batched(args) -> dict:
     if args.batched_cpu_steps > 0:
         from oracle import oracle2d
 
-        threads = min(args.cpu_threads or len(os.sched_getaffinity(0)), 16)
+        threads = cpu_threads(args)
         o = oracle2d.OracleBatch(spec, threads=threads)
         c0 = time.perf_counter()
         o.step(spec.dt, args.batched_cpu_steps)
         ct = time.perf_counter() - c0
         o.close()
         cpu = {"value": n * args.batched_cpu_steps / ct, "unit": "particle-updates/s",
-               "cores": threads, "kind": "port",
+               "cores": threads, "kind": "port", "cpu": cpu_info(),
                "sample": f"{args.batched_cpu_steps} step(s) of the same 256-scene batch from "
                          f"its initial state, fp32 oracle, OpenMP over scenes, {ct:.2f} s"}
         print(f"bench: batched CPU oracle {threads} threads: {ct:.2f} s", flush=True)
@@ -302,11 +304,36 @@ def measure_batched(args) -> dict:
     }
 
 
+def cpu_info() -> dict:
+    """Host cores this process may use and the CPU model (/proc/cpuinfo)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"affinity_cores": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_threads(args) -> int:
+    """Threads of the CPU baselines: --cpu-threads, else every core in the
+    affinity mask, bounded by OMP_NUM_THREADS when the environment sets it
+    (the host's share of cores per GPU)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
 def measure_adjoint(args) -> dict:
     """fwd+adjoint / fwd on the differentiable workload: 65 536 particles +
     512^2 heightfield, checkpoint every 16 steps.  fwd = the same steps
-    launched eagerly without a tape; fwd+adjoint = checkpointed forward +
-    adjoint sweep (segment recompute included) for an MSE height loss."""
+    without a tape, launched as hipGraph replays (the ratio's denominator)
+    and eagerly; fwd+adjoint = checkpointed forward + adjoint sweep (segment
+    recompute included) for an MSE height loss."""
     import numpy as np
 
     from coupledwateranimation_amd import configs
@@ -315,13 +342,17 @@ def measure_adjoint(args) -> dict:
     spec = configs.dam_break_diff(steps=args.adjoint_steps)
     steps, every = args.adjoint_steps, 16
     scene = spec.make_scene()
-    scene.step(spec.dt, 2)  # warm-up (kernels loaded, caches warm)
-    scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
-    scene.sync()
-    t0 = time.perf_counter()
-    scene.step(spec.dt, steps)
-    scene.sync()
-    t_fwd = time.perf_counter() - t0
+    fwd = {}
+    for mode in ("eager", "graph", "graph"):     # the first graph run captures
+        scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+        scene.step(spec.dt, 2, graph=mode == "graph")  # warm-up (kernels loaded, caches warm)
+        scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+        scene.sync()
+        t0 = time.perf_counter()
+        scene.step(spec.dt, steps, graph=mode == "graph")
+        scene.sync()
+        fwd[mode] = time.perf_counter() - t0
+    t_fwd = fwd["graph"]
     target = np.zeros((spec.wave_size[1], spec.wave_size[0]), np.float32)
     scene.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
     tape = Tape(scene, steps, every)
@@ -333,10 +364,13 @@ def measure_adjoint(args) -> dict:
     t_fa = time.perf_counter() - t0
     tape.close()
     scene.close()
-    print(f"bench: adjoint {steps} steps: fwd {1e3 * t_fwd:.2f} ms, fwd+adjoint "
-          f"{1e3 * t_fa:.2f} ms", flush=True)
+    print(f"bench: adjoint {steps} steps: fwd {1e3 * t_fwd:.2f} ms (graph), "
+          f"{1e3 * fwd['eager']:.2f} ms (eager), fwd+adjoint {1e3 * t_fa:.2f} ms", flush=True)
     return {"workload": spec.name, "steps": steps, "ckpt_every": every,
-            "fwd_ms": 1e3 * t_fwd, "fwd_adjoint_ms": 1e3 * t_fa, "ratio": t_fa / t_fwd,
+            "fwd_ms": 1e3 * t_fwd, "fwd_eager_ms": 1e3 * fwd["eager"],
+            "fwd_adjoint_ms": 1e3 * t_fa, "ratio": t_fa / t_fwd,
+            "ratio_vs_eager_fwd": t_fa / fwd["eager"],
+            "note": "ratio = fwd+adjoint / graph-replayed forward",
             "finite": bool(np.isfinite(list(grad.values())).all())}
 
 
@@ -435,8 +469,7 @@ def main() -> int:
     if args.cpu_steps > 0:
         from oracle import oracle as O
 
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
+        threads = cpu_threads(args)
         print(f"bench: CPU oracle, {threads} threads, {args.cpu_steps} steps", flush=True)
         o = O.OracleScene(p, spec.pos, spec.vel, None, spec.wave_size, spec.height0,
                           spec.height1, threads=threads)
@@ -446,7 +479,7 @@ def main() -> int:
         ct = time.perf_counter() - c0
         o.close()
         cpu = {"value": n * args.cpu_steps / ct, "unit": "particle-updates/s",
-               "cores": threads, "kind": "port",
+               "cores": threads, "kind": "port", "cpu": cpu_info(),
                "sample": f"{args.cpu_steps} coupled steps of the same {n}-particle + "
                          f"{W}x{H} workload after 1 warm-up step, fp32 oracle, OpenMP, "
                          f"{ct:.2f} s",

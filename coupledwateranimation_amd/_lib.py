@@ -196,6 +196,10 @@ SIGNATURES = {
     "cwa_grid_build": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_float,
                                  C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
+    "cwa_grid_build_boxes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.POINTER(C.c_int64), C.c_void_p]),
     "cwa_tape_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "cwa_tape_forward": (C.c_int, [C.c_void_p, C.c_float]),
     "cwa_tape_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -173,27 +173,6 @@ __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
     return min(max((int)f, 0), dim - 1);
 }
 
-// Run structure of a wave's keys: run heads, and each lane's run head and
-// run end (exclusive, lane index).
-struct RunInfo {
-    bool head;
-    int my_head;
-    int run_end;
-};
-
-__device__ __forceinline__ RunInfo wave_runs(int c) {
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(c, 1, 64);
-    RunInfo r;
-    r.head = lane == 0 || c != prev;
-    const unsigned long long heads = __ballot(r.head);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const unsigned long long after = heads & ~upto;
-    r.run_end = after ? __ffsll((long long)after) - 1 : 64;
-    r.my_head = 63 - __clzll((long long)(heads & upto));
-    return r;
-}
-
 // ---------------------------------------------------------------------------
 // grid: hash / count
 // ---------------------------------------------------------------------------

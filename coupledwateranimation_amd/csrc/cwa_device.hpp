@@ -231,6 +231,28 @@ __device__ __forceinline__ float tex_sample(const float* __restrict__ img, int w
     return bilerp(img, w, h, u * (float)w - 0.5f, v * (float)h - 0.5f);
 }
 
+// Runs of equal keys among the 64 lanes of a wave (lane order): whether the
+// lane heads a run, the lane of its run's head and the end of its run
+// (exclusive lane index).
+struct RunInfo {
+    bool head;
+    int my_head;
+    int run_end;
+};
+
+__device__ __forceinline__ RunInfo wave_runs(int c) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(c, 1, 64);
+    RunInfo r;
+    r.head = lane == 0 || c != prev;
+    const unsigned long long heads = __ballot(r.head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long after = heads & ~upto;
+    r.run_end = after ? __ffsll((long long)after) - 1 : 64;
+    r.my_head = 63 - __clzll((long long)(heads & upto));
+    return r;
+}
+
 // Blocks are dealt to the 8 XCDs round-robin by workgroup id; remap so that
 // XCD k runs one contiguous 1/8 of the logical blocks (its L2 then holds one
 // spatial slab of the sorted particles).  Bijective for any grid size.

@@ -248,16 +248,41 @@ hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st,
 }
 
 // ---------------------------------------------------------------------------
-// Uniform grid: cell hash + count (ugrid_particles_cs.glsl:90-103).  The
-// atomic return value is a provisional rank; the stable order is restored
-// by k_stable_reorder.
+// Uniform grid: a stable counting sort of keyed items (particle slots, or
+// (box, cell) pairs).  Replaces the atomicAdd insertion of
+// ugrid_particles_cs.glsl:90-117 (whose order within a cell is not
+// deterministic) with the UniformGrid2D::Build order (UniformGrid2D.cpp:
+// 75-91: items ascending within each cell):
+//   count    one atomicAdd per run of equal keys in a wave (lane order), by
+//            the run's first lane, which keeps the run's provisional base;
+//            the other lanes keep minus their distance to the run head
+//   scan     exclusive scan of the counts -> offsets
+//   scatter  each run head writes (first item, length) at its provisional
+//            place in the cell's segment; the runs tile the segment
+//   reorder  the run head walks its cell's run records (one per run, not one
+//            per item) and counts the items of the runs that start before
+//            its own; the lanes of the run take that rank plus their
+//            distance to the head.  The particle state moves to its sorted
+//            slot in the same pass.
+// The wave layout (blocks of 256 over the same item range) is identical in
+// the three passes, so every pass sees the same runs.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void count_runs(int c, bool valid, int* __restrict__ rec,
+                                           int* __restrict__ count, int i) {
+    const int lane = threadIdx.x & 63;
+    const RunInfo r = wave_runs(c);
+    int base = 0;
+    if (r.head && valid) base = atomicAdd(&count[c], r.run_end - lane);
+    if (valid) rec[i] = r.head ? base : -(lane - r.my_head);
+}
+
+// Cell hash + count of particle slots (ComputeCellIndex / Index / ComputeCounts,
+// ugrid_particles_cs.glsl:90-108, with the y-fastest linearisation).
 __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ pos, int n,
                                                     GridDesc g, int* __restrict__ key,
-                                                    int* __restrict__ trank,
+                                                    int* __restrict__ rec,
                                                     int* __restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     int c = -1;
     if (valid) {
@@ -268,59 +293,131 @@ __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ p
         c = lin_cell(g.dims, ix, iy, iz);
         key[i] = c;
     }
-    // Wave-aggregated atomics: the slots of a wave are nearly sorted by cell,
-    // so each run of equal keys (in lane order) takes one atomicAdd for its
-    // whole length, issued by the run's first lane; every run of the wave
-    // issues in the same instruction.  A lane's provisional rank is the run
-    // base plus its distance from the run head.
-    const int prev = __shfl_up(c, 1, 64);
-    const bool head = lane == 0 || c != prev;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const unsigned long long after = heads & ~upto;
-    const int run_end = after ? __ffsll((long long)after) - 1 : 64;
-    int base = 0;
-    if (head && valid) base = atomicAdd(&count[c], run_end - lane);
-    const int my_head = 63 - __clzll((long long)(heads & upto));
-    base = __shfl(base, my_head, 64);
-    const int rank = base + (lane - my_head);
-    if (valid) trank[i] = rank;
+    count_runs(c, valid, rec, count, i);
 }
 
-// InsertParticle (ugrid_particles_cs.glsl:110-117) into a provisional list.
-__global__ void k_scatter_tmp(const int* __restrict__ key, const int* __restrict__ trank,
-                              const int* __restrict__ offset, int n, int* __restrict__ tmp) {
+// Count of items whose keys are already known.
+__global__ void __launch_bounds__(256) k_key_count(const int* __restrict__ key, int n,
+                                                   int* __restrict__ rec,
+                                                   int* __restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    tmp[offset[key[i]] + trank[i]] = i;
+    const bool valid = i < n;
+    count_runs(valid ? key[i] : -1, valid, rec, count, i);
 }
 
-// Stable insertion (UniformGrid2D::Build order, UniformGrid2D.cpp:75-91):
-// the rank of slot s inside its cell is the number of members of the cell
-// with a smaller slot, counted over the provisional list.  The particle
-// state is moved to its sorted slot in the same pass.
-__global__ void k_stable_reorder(const int* __restrict__ key, const int* __restrict__ offset,
-                                 const int* __restrict__ tmp, int n, int* __restrict__ list,
-                                 const float4* __restrict__ pos_in,
-                                 const float4* __restrict__ vel_in,
-                                 const float4* __restrict__ force_in,
-                                 const int* __restrict__ id_in, float4* __restrict__ pos_out,
-                                 float4* __restrict__ vel_out, float4* __restrict__ force_out,
-                                 int* __restrict__ id_out) {
+// Run records: (first item, length) at the run's provisional place.
+__global__ void __launch_bounds__(256) k_scatter_runs(const int* __restrict__ key,
+                                                      const int* __restrict__ rec,
+                                                      const int* __restrict__ offset, int n,
+                                                      int2* __restrict__ runs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    const int c = valid ? key[i] : -1;
+    const RunInfo r = wave_runs(c);
+    if (valid && r.head) runs[offset[c] + rec[i]] = make_int2(i, r.run_end - lane);
+}
+
+// Stable destination of every item; list[d] = item_of[item] (or the item
+// itself), and the particle state moves when pos_out is set.
+__global__ void __launch_bounds__(256) k_stable_reorder(
+    const int* __restrict__ key, const int* __restrict__ rec, const int* __restrict__ offset,
+    const int2* __restrict__ runs, int n, const int* __restrict__ item_of,
+    int* __restrict__ list, const float4* __restrict__ pos_in,
+    const float4* __restrict__ vel_in, const float4* __restrict__ force_in,
+    const int* __restrict__ id_in, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+    float4* __restrict__ force_out, int* __restrict__ id_out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int c = key[s];
-    const int b = offset[c], e = offset[c + 1];
-    int r = 0;
-    for (int k = b; k < e; ++k) r += (tmp[k] < s) ? 1 : 0;
-    const int d = b + r;
-    list[d] = s;
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    const int c = valid ? key[s] : -1;
+    const RunInfo r = wave_runs(c);
+    int rank = 0, b = 0;
+    if (valid && r.head) {
+        b = offset[c];
+        const int e = offset[c + 1];
+        for (int p = b; p < e;) {
+            const int2 u = runs[p];
+            if (u.x < s) rank += u.y;
+            p += max(u.y, 1);
+        }
+        rank += b;
+    }
+    const int d = __shfl(rank, r.my_head, 64) + (lane - r.my_head);
+    if (!valid || d < 0 || d >= n) return;  // out of range only on inconsistent run records
+    list[d] = item_of ? item_of[s] : s;
     if (pos_out) {
         pos_out[d] = pos_in[s];
         vel_out[d] = vel_in[s];
         force_out[d] = force_in[s];
         id_out[d] = id_in[s];
     }
+}
+
+// Boxes (UniformGridPbdGpu3D, uniform_grid_pbd_3D_cs.glsl:145-206): the
+// cells a box overlaps are ComputeCellIndex(x - hw) .. ComputeCellIndex(x + hw)
+// per axis, with a cell size per axis.
+struct BoxGrid {
+    float gmin[3];
+    float cell[3];
+    int dims[3];
+};
+
+__device__ __forceinline__ void box_cells(const BoxGrid& g, float4 x, float4 hw, int lo[3],
+                                          int hi[3]) {
+#pragma clang fp contract(off)
+    const float a[3] = {x.x - hw.x, x.y - hw.y, x.z - hw.z};
+    const float b[3] = {x.x + hw.x, x.y + hw.y, x.z + hw.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = cell_axis(a[k], g.gmin[k], g.cell[k], g.dims[k]);
+        hi[k] = cell_axis(b[k], g.gmin[k], g.cell[k], g.dims[k]);
+    }
+}
+
+// Number of cells each box overlaps (0 for a box whose corners cross).
+__global__ void __launch_bounds__(256) k_box_cover(const float4* __restrict__ x,
+                                                   const float4* __restrict__ hw, int n,
+                                                   BoxGrid g, int* __restrict__ ncov) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    box_cells(g, x[i], hw[i], lo, hi);
+    long long m = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m *= (long long)max(hi[k] - lo[k] + 1, 0);
+    ncov[i] = (int)m;
+}
+
+// 64-bit sum of n ints (one atomic per wave), for totals an int scan would wrap.
+__global__ void __launch_bounds__(256) k_sum_i64(const int* __restrict__ a, int n,
+                                                 unsigned long long* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long v = i < n ? (long long)a[i] : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
+}
+
+// The (box, cell) pairs in box order, the cells of a box x-outer, then y,
+// then z (InsertBox's loop order): pair p = first[i] + t.
+__global__ void __launch_bounds__(256) k_box_pairs(const float4* __restrict__ x,
+                                                   const float4* __restrict__ hw, int n,
+                                                   BoxGrid g, const int* __restrict__ first,
+                                                   int* __restrict__ pkey,
+                                                   int* __restrict__ pbox) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    box_cells(g, x[i], hw[i], lo, hi);
+    int p = first[i];
+    for (int a = lo[0]; a <= hi[0]; ++a)
+        for (int b = lo[1]; b <= hi[1]; ++b)
+            for (int c = lo[2]; c <= hi[2]; ++c) {
+                pkey[p] = lin_cell(g.dims, a, b, c);
+                pbox[p] = i;
+                ++p;
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -904,28 +1001,68 @@ hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* trank,
+hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* rec,
                              int* count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(pos, n, g, key, trank, count);
+    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(pos, n, g, key, rec, count);
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offset, int n,
-                              int* tmp, hipStream_t st) {
+hipError_t launch_key_count(const int* key, int n, int* rec, int* count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_scatter_tmp<<<blocks_for(n, 256), 256, 0, st>>>(key, trank, offset, n, tmp);
+    k_key_count<<<blocks_for(n, 256), 256, 0, st>>>(key, n, rec, count);
     return hipGetLastError();
 }
 
-hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
-                                 int* list, const ParticleArrays* in, ParticleArrays* out,
-                                 hipStream_t st) {
+hipError_t launch_scatter_runs(const int* key, const int* rec, const int* offset, int n,
+                               int2* runs, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_scatter_runs<<<blocks_for(n, 256), 256, 0, st>>>(key, rec, offset, n, runs);
+    return hipGetLastError();
+}
+
+hipError_t launch_stable_reorder(const int* key, const int* rec, const int* offset,
+                                 const int2* runs, int n, const int* item_of, int* list,
+                                 const ParticleArrays* in, ParticleArrays* out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_stable_reorder<<<blocks_for(n, 256), 256, 0, st>>>(
-        key, offset, tmp, n, list, in ? in->pos : nullptr, in ? in->vel : nullptr,
-        in ? in->force : nullptr, in ? in->id : nullptr, out ? out->pos : nullptr,
-        out ? out->vel : nullptr, out ? out->force : nullptr, out ? out->id : nullptr);
+        key, rec, offset, runs, n, item_of, list, in ? in->pos : nullptr,
+        in ? in->vel : nullptr, in ? in->force : nullptr, in ? in->id : nullptr,
+        out ? out->pos : nullptr, out ? out->vel : nullptr, out ? out->force : nullptr,
+        out ? out->id : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_cover(const float4* x, const float4* hw, int n, const float gmin[3],
+                            const float cell[3], const int dims[3], int* ncov, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    BoxGrid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.gmin[k] = gmin[k];
+        g.cell[k] = cell[k];
+        g.dims[k] = dims[k];
+    }
+    k_box_cover<<<blocks_for(n, 256), 256, 0, st>>>(x, hw, n, g, ncov);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_i64(const int* a, int n, unsigned long long* out, hipStream_t st) {
+    k_fill_i32<<<1, 64, 0, st>>>(reinterpret_cast<int*>(out), 2, 0);
+    if (n > 0) k_sum_i64<<<blocks_for(n, 256), 256, 0, st>>>(a, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_pairs(const float4* x, const float4* hw, int n, const float gmin[3],
+                            const float cell[3], const int dims[3], const int* first,
+                            int* pkey, int* pbox, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    BoxGrid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.gmin[k] = gmin[k];
+        g.cell[k] = cell[k];
+        g.dims[k] = dims[k];
+    }
+    k_box_pairs<<<blocks_for(n, 256), 256, 0, st>>>(x, hw, n, g, first, pkey, pbox);
     return hipGetLastError();
 }
 

@@ -28,14 +28,26 @@ size_t scan_workspace_ints(int n);
 hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st,
                        int* zero_out = nullptr);
 hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st);
-hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* trank,
+// Stable counting sort of keyed items (rec: n ints, runs: n int2 scratch).
+hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* rec,
                              int* count, hipStream_t st);
-hipError_t launch_scatter_tmp(const int* key, const int* trank, const int* offset, int n,
-                              int* tmp, hipStream_t st);
-// in/out may be null: only the index list is produced.
-hipError_t launch_stable_reorder(const int* key, const int* offset, const int* tmp, int n,
-                                 int* list, const ParticleArrays* in, ParticleArrays* out,
-                                 hipStream_t st);
+hipError_t launch_key_count(const int* key, int n, int* rec, int* count, hipStream_t st);
+hipError_t launch_scatter_runs(const int* key, const int* rec, const int* offset, int n,
+                               int2* runs, hipStream_t st);
+// list[d] = item_of ? item_of[s] : s at each item's stable destination d;
+// in/out may be null: only the list is produced.
+hipError_t launch_stable_reorder(const int* key, const int* rec, const int* offset,
+                                 const int2* runs, int n, const int* item_of, int* list,
+                                 const ParticleArrays* in, ParticleArrays* out, hipStream_t st);
+// Boxes: cells covered per box (ncov, n ints), then the (box, cell) pairs at
+// first[i] = exclusive scan of ncov (pkey: cell, pbox: box index).
+hipError_t launch_box_cover(const float4* x, const float4* hw, int n, const float gmin[3],
+                            const float cell[3], const int dims[3], int* ncov, hipStream_t st);
+// *out = sum of a[0..n) in 64 bits (non-negative values).
+hipError_t launch_sum_i64(const int* a, int n, unsigned long long* out, hipStream_t st);
+hipError_t launch_box_pairs(const float4* x, const float4* hw, int n, const float gmin[3],
+                            const float cell[3], const int dims[3], const int* first,
+                            int* pkey, int* pbox, hipStream_t st);
 // nbw (kNbEntries*n uint2, neighbour words) and nrec (2*n float4) are written by the density pass
 // and read by the force pass.  rho_raw (density before the wave term and the
 // rest-density clamp) and fpre (force before the foam branch) are optional

@@ -148,7 +148,8 @@ class UniformGrid3D {
 
    private:
     GridDesc g_{};
-    DeviceBuffer<int> key_, trank_, count_[2], tmp_, list_, offset_, ws_;
+    DeviceBuffer<int> key_, rec_, count_[2], list_, offset_, ws_;
+    DeviceBuffer<int2> runs_;
     int cur_ = 0, last_ = 0;
 };
 
@@ -285,6 +286,12 @@ class CoupledScene : public Module {
     void NeighbourCounts(int32_t* out);
     hipStream_t stream() const { return st_; }
     const DevParams& devparams() const { return dp_; }
+    // Layout generation: advanced by every Reinit (particle count, heightfield
+    // size) and by every SetParams that changes the grid, the sign of the
+    // coupling band or the wake mode.  Objects that cache buffer sizes or
+    // captured code paths of this scene (tapes, fits) record it and refuse
+    // to run once it has moved.
+    unsigned long long layout_generation() const { return layout_gen_; }
     // Host-side phase of the buffer rotations (wave triple buffer x grid
     // counter double buffer), period 6.
     int phase() const { return wave.phase() * 2 + grid.parity(); }
@@ -323,6 +330,7 @@ class CoupledScene : public Module {
     int graph_phase_ = -1;
     float dt_ = 0.0f;
     bool stepped_ = false;  // a step has run since the last (re)init
+    unsigned long long layout_gen_ = 0;
     DeviceBuffer<int> flag_;
 };
 

@@ -62,8 +62,8 @@ void UniformGrid3D::Init(const GridDesc& g, int max_particles) {
     const size_t nc = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
     const size_t np = (size_t)std::max(max_particles, 1);
     key_.resize(np);
-    trank_.resize(np);
-    tmp_.resize(np);
+    rec_.resize(np);
+    runs_.resize(np);
     list_.resize(np);
     count_[0].resize(nc);
     count_[1].resize(nc);
@@ -88,7 +88,7 @@ void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in
     };
     int* cnt = count_[cur_].get();  // zero: cleared by the previous build's scan
     mark(K_HASH, 0);
-    hip_check(launch_hash_count(pos, n, g_, key_.get(), trank_.get(), cnt, st), "hash");
+    hip_check(launch_hash_count(pos, n, g_, key_.get(), rec_.get(), cnt, st), "hash");
     mark(K_HASH, 1);
     mark(K_SCAN, 0);
     hip_check(launch_scan(cnt, off, nc, ws_.get(), st, count_[cur_ ^ 1].get()), "scan");
@@ -96,10 +96,11 @@ void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in
     last_ = cur_;
     cur_ ^= 1;
     mark(K_SCATTER, 0);
-    hip_check(launch_scatter_tmp(key_.get(), trank_.get(), off, n, tmp_.get(), st), "scatter");
+    hip_check(launch_scatter_runs(key_.get(), rec_.get(), off, n, runs_.get(), st), "scatter");
     mark(K_SCATTER, 1);
     mark(K_REORDER, 0);
-    hip_check(launch_stable_reorder(key_.get(), off, tmp_.get(), n, lst, in, out, st),
+    hip_check(launch_stable_reorder(key_.get(), rec_.get(), off, runs_.get(), n, nullptr, lst, in,
+                                    out, st),
               "reorder");
     mark(K_REORDER, 1);
 }
@@ -397,6 +398,7 @@ void CoupledScene::Reinit(const cwa_init& init) {
     invalidate_graph();
     hip_check(hipStreamSynchronize(st_), "sync");
     stepped_ = false;
+    ++layout_gen_;
     particles.Allocate(init.n);
     particles.Upload(init.pos, init.vel, init.force, st_);
     grid.Init(grid_desc(params_), init.n);
@@ -419,10 +421,13 @@ void CoupledScene::SetParams(const cwa_params& p) {
                               p.grid_min[1] != params_.grid_min[1] ||
                               p.grid_min[2] != params_.grid_min[2] ||
                               p.cell_size != params_.cell_size || ydiv_of(p) != ydiv_of(params_);
+    const bool path_changed = (p.couple_band > 0.0f) != (params_.couple_band > 0.0f) ||
+                              make_devparams(p, 1, 1).wake != make_devparams(params_, 1, 1).wake;
     invalidate_graph();
     hip_check(hipStreamSynchronize(st_), "sync");
     params_ = p;
     stepped_ = false;  // the last step's neighbour records used the old parameters
+    if (grid_changed || path_changed) ++layout_gen_;
     if (grid_changed) {
         grid.Init(grid_desc(params_), particles.n());
         grid.ClearCounters(st_);
@@ -968,18 +973,76 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
         g.cell = cell_size;
         g.cell_y = cell_size;
         const int nc = dims[0] * dims[1] * dims[2];
-        DeviceBuffer<int> trank((size_t)std::max(n, 1)), tmp((size_t)std::max(n, 1)),
-            ws(scan_workspace_ints(nc));
+        DeviceBuffer<int> rec((size_t)std::max(n, 1)), ws(scan_workspace_ints(nc));
+        DeviceBuffer<int2> runs((size_t)std::max(n, 1));
         hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
         hip_check(launch_hash_count(reinterpret_cast<const float4*>(d_pos4), n, g, d_key,
-                                    trank.get(), d_count, st),
+                                    rec.get(), d_count, st),
                   "hash");
         hip_check(launch_scan(d_count, d_offset, nc, ws.get(), st), "scan");
-        hip_check(launch_scatter_tmp(d_key, trank.get(), d_offset, n, tmp.get(), st), "scatter");
-        hip_check(launch_stable_reorder(d_key, d_offset, tmp.get(), n, d_list, nullptr, nullptr,
-                                        st),
+        hip_check(launch_scatter_runs(d_key, rec.get(), d_offset, n, runs.get(), st), "scatter");
+        hip_check(launch_stable_reorder(d_key, rec.get(), d_offset, runs.get(), n, nullptr, d_list,
+                                        nullptr, nullptr, st),
                   "stable insert");
         hip_check(hipStreamSynchronize(st), "grid sync");
+    });
+}
+
+int cwa_grid_build_boxes(const float* d_centre4, const float* d_halfw4, int n,
+                         const float grid_min[3], const float cell_size[3],
+                         const int32_t dims[3], int32_t* d_count, int32_t* d_offset,
+                         int32_t* d_list, int64_t list_capacity, int64_t* total,
+                         void* stream) {
+    return guarded([&] {
+        CWA_REQUIRE(n >= 0, "n < 0");
+        CWA_REQUIRE(grid_min && cell_size && dims && d_count && d_offset && total,
+                    "null argument");
+        CWA_REQUIRE(n == 0 || (d_centre4 && d_halfw4), "null argument");
+        CWA_REQUIRE(list_capacity >= 0 && (list_capacity == 0 || d_list), "null list");
+        for (int a = 0; a < 3; ++a) {
+            CWA_REQUIRE(cell_size[a] > 0.0f && std::isfinite(cell_size[a]),
+                        "cell_size must be > 0");
+            CWA_REQUIRE(dims[a] >= 1, "dims must be >= 1");
+        }
+        CWA_REQUIRE((double)dims[0] * dims[1] * dims[2] <= (double)(1 << 30), "too many cells");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int nc = dims[0] * dims[1] * dims[2];
+        const float4* x = reinterpret_cast<const float4*>(d_centre4);
+        const float4* hw = reinterpret_cast<const float4*>(d_halfw4);
+        // cells per box, then the pair index of each box's first pair
+        DeviceBuffer<int> ncov((size_t)std::max(n, 1)), first((size_t)n + 1),
+            ws(scan_workspace_ints(std::max(n, nc)));
+        DeviceBuffer<unsigned long long> sum(1);
+        hip_check(launch_box_cover(x, hw, n, grid_min, cell_size, dims, ncov.get(), st), "cover");
+        hip_check(launch_sum_i64(ncov.get(), n, sum.get(), st), "pair total");
+        unsigned long long npairs64 = 0;
+        hip_check(hipMemcpyAsync(&npairs64, sum.get(), sizeof(npairs64), hipMemcpyDeviceToHost,
+                                 st),
+                  "pair count");
+        hip_check(hipStreamSynchronize(st), "pair count sync");
+        *total = (int64_t)npairs64;
+        CWA_REQUIRE(npairs64 <= (unsigned long long)CWA_MAX_PARTICLES,
+                    "more (box, cell) pairs than CWA_MAX_PARTICLES (returned in *total)");
+        CWA_REQUIRE((int64_t)npairs64 <= list_capacity,
+                    "list_capacity is below the number of (box, cell) pairs (returned in "
+                    "*total)");
+        const int npairs = (int)npairs64;
+        hip_check(launch_scan(ncov.get(), first.get(), n, ws.get(), st), "cover scan");
+        const size_t np = (size_t)std::max(npairs, 1);
+        DeviceBuffer<int> pkey(np), pbox(np), rec(np);
+        DeviceBuffer<int2> runs(np);
+        hip_check(launch_box_pairs(x, hw, n, grid_min, cell_size, dims, first.get(), pkey.get(),
+                                   pbox.get(), st),
+                  "pairs");
+        hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
+        hip_check(launch_key_count(pkey.get(), npairs, rec.get(), d_count, st), "count");
+        hip_check(launch_scan(d_count, d_offset, nc, ws.get(), st), "scan");
+        hip_check(launch_scatter_runs(pkey.get(), rec.get(), d_offset, npairs, runs.get(), st),
+                  "scatter");
+        hip_check(launch_stable_reorder(pkey.get(), rec.get(), d_offset, runs.get(), npairs,
+                                        pbox.get(), d_list, nullptr, nullptr, st),
+                  "stable insert");
+        hip_check(hipStreamSynchronize(st), "box grid sync");
     });
 }
 

@@ -72,6 +72,13 @@ class Tape {
     void record_step(std::vector<StepTape>& set, int k, const Ckpt& c);
     void adjoint_step(const StepTape& t, hipStream_t st);
     void reduce_partials(hipStream_t st);
+    // The scene must still have the layout the tape was sized for.
+    void check_layout() const {
+        if (S_.layout_generation() != gen_)
+            throw Error(CWA_ESTATE,
+                        "the scene was reinitialised or its grid, coupling band or wave mode "
+                        "changed after the tape was created");
+    }
     template <class T>
     void copy(T* dst, const T* src, size_t count) {
         if (count)
@@ -81,6 +88,7 @@ class Tape {
 
     CoupledScene& S_;
     hipStream_t st_;
+    unsigned long long gen_;
     int nsteps_, every_, nseg_;
     int n_, W_, H_, ncell_, ncol_;
     float dt_ = 0.0f;
@@ -95,7 +103,7 @@ class Tape {
 };
 
 Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
-    : S_(s), st_(s.stream()) {
+    : S_(s), st_(s.stream()), gen_(s.layout_generation()) {
     if (nsteps < 1) throw Error(CWA_EINVAL, "nsteps must be >= 1");
     if (every < 1) throw Error(CWA_EINVAL, "ckpt_every must be >= 1");
     if (s.params().torque_coeff != 0.0f)
@@ -122,6 +130,18 @@ Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
     }
     tp_.resize(std::min(every, nsteps));
     for (auto& t : tp_) alloc_step_tape(t);
+    // the stream and events are created last, so an allocation failure
+    // above leaves nothing behind (the buffers free themselves); the
+    // constructor body after them releases them if it throws
+    for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
+    adjrec_.resize(2 * n);
+    for (auto* b : {&G0_, &G1_, &G0n_, &G1n_}) b->resize(tex);
+    rhob_.resize(n);
+    rtb_.resize(n);
+    Sbar_.resize(ncol_);
+    pgrad_.resize(GRAD_NUM);
+    acc_wave_.resize((size_t)adj_wave_blocks(s.devparams()) * 5);
+    acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
     if (pipelined) {
         if (nseg_ > 1) {
             tp2_.resize(tp_.size());
@@ -136,15 +156,6 @@ Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
             throw;
         }
     }
-    for (auto* b : {&xb_, &vb_, &xb2_, &vb2_, &xl_, &vl_}) b->resize(n);
-    adjrec_.resize(2 * n);
-    for (auto* b : {&G0_, &G1_, &G0n_, &G1n_}) b->resize(tex);
-    rhob_.resize(n);
-    rtb_.resize(n);
-    Sbar_.resize(ncol_);
-    pgrad_.resize(GRAD_NUM);
-    acc_wave_.resize((size_t)adj_wave_blocks(s.devparams()) * 5);
-    acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
 }
 
 Tape::~Tape() { release_pipeline(); }
@@ -211,6 +222,7 @@ void Tape::SaveInitial() {
 }
 
 void Tape::ForwardFromInitial(float dt) {
+    check_layout();
     dt_ = dt;
     restore(ck_[0]);
     int done = 0;
@@ -235,6 +247,7 @@ void Tape::SeedMse(const float* target, double* partials, int nblocks, double* l
 }
 
 void Tape::BackwardSeeded() {
+    check_layout();
     if (!forward_done_) throw Error(CWA_ESTATE, "tape backward before forward");
     auto zero = [&](DeviceBuffer<double>& b) {
         hip_check(launch_fill_i32(reinterpret_cast<int*>(b.get()), (int)(2 * b.size()), 0, st_),
@@ -265,6 +278,7 @@ void Tape::reduce_partials(hipStream_t st) {
 }
 
 void Tape::Forward(float dt) {
+    check_layout();
     dt_ = dt;
     int done = 0;
     for (int seg = 0; seg < nseg_; ++seg) {
@@ -336,6 +350,7 @@ void Tape::adjoint_step(const StepTape& t, hipStream_t st) {
 
 void Tape::Backward(const float* dH0, const float* dH1, const float* dpos, const float* dvel,
                     double* grad, float* dpos0, float* dvel0) {
+    check_layout();
     if (!forward_done_) throw Error(CWA_ESTATE, "tape backward before forward");
     const size_t tex = (size_t)W_ * H_;
     // seeds: heights as given, particle seeds permuted into the final slots
@@ -447,6 +462,50 @@ class FitBatch {
             for (int q = 0; q < s; ++q)
                 if (sc_[q] == sc_[s]) throw Error(CWA_EINVAL, "a scene appears twice");
         }
+        try {
+            Build(targets, nsteps, every, dt);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+
+    ~FitBatch() { release(); }
+
+    void Run(double* losses, double* grads) {
+        const int ns = (int)sc_.size();
+        for (int s = 0; s < ns; ++s)
+            if (sc_[s]->layout_generation() != gen_[s])
+                throw Error(CWA_ESTATE,
+                            "a fit scene was reinitialised or its grid, coupling band or wave "
+                            "mode changed after the fit was created");
+        hip_check(hipGraphLaunch(exec_, ms_), "fit graph");
+        std::vector<double> g((size_t)ns * GRAD_NUM), l(ns);
+        for (int s = 0; s < ns; ++s)
+            hip_check(hipMemcpyAsync(g.data() + (size_t)s * GRAD_NUM, tapes_[s]->grad_dev(),
+                                     sizeof(double) * GRAD_NUM, hipMemcpyDeviceToHost, ms_),
+                      "grads");
+        hip_check(hipMemcpyAsync(l.data(), loss_.get(), sizeof(double) * ns,
+                                 hipMemcpyDeviceToHost, ms_),
+                  "losses");
+        hip_check(hipStreamSynchronize(ms_), "fit");
+        // the device state now matches the rotation the capture ended in
+        for (int s = 0; s < ns; ++s) sc_[s]->set_rotation(rot_end_[s]);
+        if (losses) std::memcpy(losses, l.data(), sizeof(double) * ns);
+        if (grads) std::memcpy(grads, g.data(), sizeof(double) * g.size());
+    }
+
+    int nscenes() const { return (int)sc_.size(); }
+    long long TapeBytes(int s) const {
+        if (s < 0 || s >= (int)sc_.size()) throw Error(CWA_EINVAL, "scene index out of range");
+        return bytes_[s];
+    }
+
+   private:
+    void Build(const float* const* targets, int nsteps, int every, float dt) {
+        const int ns = (int)sc_.size();
+        gen_.resize(ns);
+        for (int s = 0; s < ns; ++s) gen_[s] = sc_[s]->layout_generation();
         hip_check(hipStreamCreateWithFlags(&ms_, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "event");
         ev_.assign(ns, nullptr);
@@ -472,42 +531,21 @@ class FitBatch {
         Capture(dt);
     }
 
-    ~FitBatch() {
+    void release() {
         if (exec_) (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
         if (ms_) {
             (void)hipStreamSynchronize(ms_);
             (void)hipStreamDestroy(ms_);
+            ms_ = nullptr;
         }
-        for (auto e : ev_)
+        for (auto& e : ev_)
             if (e) (void)hipEventDestroy(e);
+        ev_.clear();
         if (fork_) (void)hipEventDestroy(fork_);
+        fork_ = nullptr;
     }
 
-    void Run(double* losses, double* grads) {
-        hip_check(hipGraphLaunch(exec_, ms_), "fit graph");
-        const int ns = (int)sc_.size();
-        std::vector<double> g((size_t)ns * GRAD_NUM), l(ns);
-        for (int s = 0; s < ns; ++s)
-            hip_check(hipMemcpyAsync(g.data() + (size_t)s * GRAD_NUM, tapes_[s]->grad_dev(),
-                                     sizeof(double) * GRAD_NUM, hipMemcpyDeviceToHost, ms_),
-                      "grads");
-        hip_check(hipMemcpyAsync(l.data(), loss_.get(), sizeof(double) * ns,
-                                 hipMemcpyDeviceToHost, ms_),
-                  "losses");
-        hip_check(hipStreamSynchronize(ms_), "fit");
-        // the device state now matches the rotation the capture ended in
-        for (int s = 0; s < ns; ++s) sc_[s]->set_rotation(rot_end_[s]);
-        if (losses) std::memcpy(losses, l.data(), sizeof(double) * ns);
-        if (grads) std::memcpy(grads, g.data(), sizeof(double) * g.size());
-    }
-
-    int nscenes() const { return (int)sc_.size(); }
-    long long TapeBytes(int s) const {
-        if (s < 0 || s >= (int)sc_.size()) throw Error(CWA_EINVAL, "scene index out of range");
-        return bytes_[s];
-    }
-
-   private:
     void Capture(float dt) {
         const int ns = (int)sc_.size();
         for (int s = 0; s < ns; ++s) hip_check(hipStreamSynchronize(sc_[s]->stream()), "sync");
@@ -543,6 +581,7 @@ class FitBatch {
     std::vector<DeviceBuffer<float>> target_;
     std::vector<long long> bytes_;
     std::vector<CoupledScene::Rotation> rot_end_;
+    std::vector<unsigned long long> gen_;
     DeviceBuffer<double> part_, loss_;
     hipStream_t ms_ = nullptr;
     hipEvent_t fork_ = nullptr;

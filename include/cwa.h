@@ -209,9 +209,11 @@ int cwa_scene_get_params(const cwa_scene* s, cwa_params* p);
 /* ---- heightfield Module: StencilImage2DTripleBuffered ---- */
 /* Reinit (.cpp:42-59): run MODE_INIT twice, both history levels equal. */
 int cwa_scene_wave_reinit(cwa_scene* s);
-/* ReinitFromTexture (.cpp:61-77): both history levels get texel (i, j) of
- * a single-channel fp32 host image read at (2*i, j) (texelFetch at
- * coord*(2,1), wave_comp.glsl:76-80); needs tex_w >= 2*w, tex_h >= h. */
+/* ReinitFromTexture (.cpp:61-77): one MODE_INIT_FROM_TEXTURE dispatch into
+ * the write image, then PingPong, so only the newest level gets texel
+ * (i, j) of a single-channel fp32 host image read at (2*i, j) (texelFetch
+ * at coord*(2,1), wave_comp.glsl:76-80) and the previous newest becomes the
+ * previous level; fetches past the image read 0. */
 int cwa_scene_wave_reinit_from_texture(cwa_scene* s, const float* tex, int tex_w, int tex_h);
 /* Compute (.cpp:79-95): one EVOLVE dispatch then PingPong, with no
  * particle coupling. */
@@ -262,6 +264,24 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
                    const int32_t dims[3], int32_t* d_key, int32_t* d_count,
                    int32_t* d_offset, int32_t* d_list, void* stream);
 
+/* UniformGridPbdGpu3D::CollisionQuery's grid build (UniformGridGpu3D.cpp:
+ * 121-162; uniform_grid_pbd_3D_cs.glsl ComputeCounts / ComputeCellIndex /
+ * Index / InsertBox :145-206): box i spans the cells
+ * ComputeCellIndex(centre - halfw) .. ComputeCellIndex(centre + halfw) per
+ * axis, cell = clamp(floor((p - grid_min) / cell_size), 0, dims - 1) with a
+ * cell size per axis, linear cell (ix*dims_z + iz)*dims_y + iy.  count[c]
+ * = boxes overlapping cell c, offset = its exclusive scan (num_cells+1),
+ * list[offset[c] ..] = those boxes in ascending index order (the reference
+ * inserts with atomicAdd, in no fixed order).  d_centre4 / d_halfw4: float4
+ * per box (xyz used).  *total receives the number of (box, cell) pairs
+ * (offset[num_cells]); list must hold that many (the reference sizes it 8
+ * per box, boxes no larger than a cell), else CWA_EINVAL with *total set. */
+int cwa_grid_build_boxes(const float* d_centre4, const float* d_halfw4, int n,
+                         const float grid_min[3], const float cell_size[3],
+                         const int32_t dims[3], int32_t* d_count, int32_t* d_offset,
+                         int32_t* d_list, int64_t list_capacity, int64_t* total,
+                         void* stream);
+
 /* ---- gradients: checkpointed forward + adjoint (no reference
  *      counterpart; the differentiable form of the step above) ----
  * The adjoint differentiates the step as run, with every discrete decision
@@ -283,7 +303,10 @@ typedef struct cwa_grad {
 typedef struct cwa_tape cwa_tape;
 
 /* A tape over `nsteps` steps of scene s from its current state, with a
- * checkpoint every `ckpt_every` steps.  s must outlive the tape.  Device
+ * checkpoint every `ckpt_every` steps.  s must outlive the tape, and its
+ * layout must stay as it was at creation: after cwa_scene_reinit, or a
+ * cwa_scene_set_params that changes the grid, the sign of couple_band or
+ * the wake mode, the tape's calls return CWA_ESTATE.  Device
  * memory: nsteps / ckpt_every + 1 checkpoints of the particle state and two
  * heightfields, and two step tapes of ckpt_every steps each (one recorded
  * while the adjoint walks the other). */
@@ -311,7 +334,9 @@ int cwa_tape_destroy(cwa_tape* t);
  * its own stream) at creation; cwa_fit_run replays it and returns each
  * scene's loss and parameter gradients.  cwa_scene_set_params between runs
  * takes effect without a new capture as long as the grid, the coupling
- * band and the wave mode stay as they were.  Scenes need torque_coeff == 0
+ * band's sign and the wave mode stay as they were (otherwise, and after a
+ * cwa_scene_reinit of a fit scene, cwa_fit_run returns CWA_ESTATE).
+ * Scenes need torque_coeff == 0
  * and check_finite == 0, must not be stepped or destroyed while the fit
  * exists, and each scene may appear once. */
 typedef struct cwa_fit cwa_fit;

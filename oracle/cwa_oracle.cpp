@@ -98,6 +98,12 @@ void cwao_exclusive_scan(const int32_t* in, int n, int32_t* out) { exclusive_sca
 
 void cwao_blelloch_scan(int32_t* x, int n) { blelloch_scan(x, n); }
 
+long long cwao_grid_build_boxes(const float* x4, const float* hw4, int n, const float* gmin,
+                                const float* cell, const int32_t* dims, int32_t* count,
+                                int32_t* offset, int32_t* list, long long capacity) {
+    return grid_build_boxes(x4, hw4, n, gmin, cell, dims, count, offset, list, capacity);
+}
+
 int cwao_grid_build(const cwa_params* P, const float* pos4, int n, int32_t* key, int32_t* count,
                     int32_t* offset, int32_t* list) {
     const Consts<float> c = make_consts<float>(*P, 1, 1);

@@ -20,6 +20,8 @@
 //                            and the reference's Blelloch tree restated
 //   grid build ............. pinned: permutation / stability invariants,
 //                            brute-force neighbour sets
+//   box grid build ......... pinned: brute-force interval overlap of every
+//                            box with every cell, clamping at the edges
 //   density / EOS .......... pinned: isolated particle, two-particle closed
 //                            form, wave term closed form
 //   forces ................. pinned: two-particle closed form, isolated
@@ -285,6 +287,57 @@ static void grid_build(const Consts<R>& c, const R* pos4, int n, int32_t* key, i
         list[offset[k] + fill[k]] = i;
         fill[k]++;
     }
+}
+
+// The box grid of UniformGridPbdGpu3D::CollisionQuery (UniformGridGpu3D.cpp:
+// 121-162): ComputeCounts and InsertBox (uniform_grid_pbd_3D_cs.glsl:145-165,
+// 180-206) over the cells ComputeCellIndex(x - hw) .. ComputeCellIndex(x + hw)
+// per axis (:167-173: floor((p - min) / cellSize) clamped to the grid, with a
+// cell size per axis), the exclusive scan of the counts (ParallelScan), and
+// the insertion made stable: boxes ascending within a cell, where the
+// shader's atomicAdd leaves the order open.  The cell is linearised y
+// fastest like the particle grid (Index() at :175-178 uses (i*ny + j)*nx + k).
+// Arithmetic in float, as the shader: x - hw and x + hw rounded, then the
+// correctly rounded division.  Returns the number of (box, cell) pairs;
+// list is written only when it holds that many (capacity).
+static inline int box_cell_axis(float p, float lo, float cell, int dim) {
+    float f = std::floor((p - lo) / cell);
+    f = std::min(std::max(f, -1.0f), (float)dim);
+    return std::min(std::max((int)f, 0), dim - 1);
+}
+
+static long long grid_build_boxes(const float* x4, const float* hw4, int n, const float gmin[3],
+                                  const float cell[3], const int dims[3], int32_t* count,
+                                  int32_t* offset, int32_t* list, long long capacity) {
+    const int nc = dims[0] * dims[1] * dims[2];
+    std::fill(count, count + nc, 0);
+    std::vector<int> lo(3 * (size_t)n), hi(3 * (size_t)n);
+    long long total = 0;
+    for (int i = 0; i < n; ++i) {
+        long long m = 1;
+        for (int a = 0; a < 3; ++a) {
+            const float p0 = x4[4 * i + a] - hw4[4 * i + a];
+            const float p1 = x4[4 * i + a] + hw4[4 * i + a];
+            lo[3 * i + a] = box_cell_axis(p0, gmin[a], cell[a], dims[a]);
+            hi[3 * i + a] = box_cell_axis(p1, gmin[a], cell[a], dims[a]);
+            m *= std::max(hi[3 * i + a] - lo[3 * i + a] + 1, 0);
+        }
+        total += m;
+        for (int a = lo[3 * i]; a <= hi[3 * i]; ++a)
+            for (int b = lo[3 * i + 1]; b <= hi[3 * i + 1]; ++b)
+                for (int c = lo[3 * i + 2]; c <= hi[3 * i + 2]; ++c) count[lin_cell(dims, a, b, c)]++;
+    }
+    exclusive_scan(count, nc, offset);
+    if (total > capacity) return total;
+    std::vector<int32_t> fill(nc, 0);
+    for (int i = 0; i < n; ++i)
+        for (int a = lo[3 * i]; a <= hi[3 * i]; ++a)
+            for (int b = lo[3 * i + 1]; b <= hi[3 * i + 1]; ++b)
+                for (int c = lo[3 * i + 2]; c <= hi[3 * i + 2]; ++c) {
+                    const int k = lin_cell(dims, a, b, c);
+                    list[offset[k] + fill[k]++] = i;
+                }
+    return total;
 }
 
 // Visit the candidates of particle slot i in sorted order: the 9 columns

@@ -39,6 +39,8 @@ def lib():
             "cwao_exclusive_scan": (None, [ip, C.c_int, ip]),
             "cwao_blelloch_scan": (None, [ip, C.c_int]),
             "cwao_grid_build": (C.c_int, [P, fp, C.c_int, ip, ip, ip, ip]),
+            "cwao_grid_build_boxes": (C.c_longlong, [fp, fp, C.c_int, fp, fp, ip, ip, ip, ip,
+                                                     C.c_longlong]),
             "cwao_neighbor_counts": (C.c_int, [P, fp, C.c_int, C.c_int, ip]),
             "cwao_init_wave_f": (None, [C.c_float, fp, C.c_int, C.c_int]),
             "cwao_init_wave_d": (None, [C.c_double, dp, C.c_int, C.c_int]),
@@ -93,6 +95,29 @@ def grid_build(params, pos4: np.ndarray) -> dict:
     lib().cwao_grid_build(C.byref(params), _p(pos4, C.c_float), n, _p(key, C.c_int32),
                           _p(count, C.c_int32), _p(offset, C.c_int32), _p(lst, C.c_int32))
     return {"key": key, "count": count, "offset": offset, "list": lst}
+
+
+def grid_build_boxes(centre4, halfw4, grid_min, cell_size, dims) -> dict:
+    """UniformGridPbdGpu3D's box grid: counts, offsets and the stable list
+    (boxes ascending within each cell)."""
+    x = np.ascontiguousarray(centre4, np.float32).reshape(-1, 4)
+    hw = np.ascontiguousarray(halfw4, np.float32).reshape(-1, 4)
+    n = x.shape[0]
+    gmin = np.ascontiguousarray(grid_min, np.float32)
+    cell = np.ascontiguousarray(cell_size, np.float32)
+    d = np.ascontiguousarray(dims, np.int32)
+    nc = int(d[0]) * int(d[1]) * int(d[2])
+    count = np.empty(nc, np.int32)
+    offset = np.empty(nc + 1, np.int32)
+    total = lib().cwao_grid_build_boxes(_p(x, C.c_float), _p(hw, C.c_float), n,
+                                        _p(gmin, C.c_float), _p(cell, C.c_float),
+                                        _p(d, C.c_int32), _p(count, C.c_int32),
+                                        _p(offset, C.c_int32), None, 0)
+    lst = np.empty(max(int(total), 1), np.int32)
+    lib().cwao_grid_build_boxes(_p(x, C.c_float), _p(hw, C.c_float), n, _p(gmin, C.c_float),
+                                _p(cell, C.c_float), _p(d, C.c_int32), _p(count, C.c_int32),
+                                _p(offset, C.c_int32), _p(lst, C.c_int32), int(total))
+    return {"count": count, "offset": offset, "list": lst[:int(total)], "total": int(total)}
 
 
 def neighbor_counts(params, pos4: np.ndarray, all_pairs: bool) -> np.ndarray:

@@ -203,3 +203,61 @@ def test_checkpoint_interval_and_autograd(gpu, cwa, oracle):
     for t, v in res[1:]:
         np.testing.assert_allclose(t, res[0][0], rtol=1e-4)
         np.testing.assert_allclose(v, res[0][1], rtol=1e-3, atol=1e-4 * np.abs(res[0][1]).max())
+
+
+def test_tape_and_fit_refuse_a_changed_scene(gpu, cwa):
+    """A tape or fit sized for a scene's layout refuses to run (CWA_ESTATE,
+    before any launch) once the scene was reinitialised or its grid, the
+    sign of its coupling band or its wave mode changed; a change of plain
+    scalars (a fit's business) keeps both usable."""
+    from coupledwateranimation_amd import configs, fit
+    from coupledwateranimation_amd.fit import FitBatch
+    from coupledwateranimation_amd.gradient import Tape
+
+    spec = configs.dam_break_diff(n=2048, wave=32, steps=4)
+    sc = spec.make_scene()
+    tape = Tape(sc, 4, 2)
+    q = spec.params.copy()
+    q.visc *= 1.5
+    sc.set_params(q)                       # scalars only: still valid
+    tape.forward(spec.dt)
+    tape.backward(dheight0=np.ones((32, 32), np.float32))
+    for change in ("grid", "band", "wake", "reinit"):
+        r = spec.params.copy()
+        if change == "grid":
+            configs.grid_for_box(r, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1.5 * r.h)
+        elif change == "band":
+            r.couple_band = 0.0
+        elif change == "wake":
+            r.wave_type = 0.5
+        if change == "reinit":
+            sc.reinit(spec.pos, spec.vel, None, spec.wave_size, spec.height0, spec.height1)
+        else:
+            sc.set_params(r)
+        with pytest.raises(cwa.CwaError) as e:
+            tape.forward(spec.dt)
+        assert e.value.code == -4, change
+        with pytest.raises(cwa.CwaError) as e:
+            tape.backward(dheight0=np.ones((32, 32), np.float32))
+        assert e.value.code == -4, change
+        tape.close()
+        sc.set_params(spec.params)
+        tape = Tape(sc, 4, 2)             # a new tape on the new layout works
+        tape.forward(spec.dt)
+    tape.close()
+    sc.close()
+
+    specs, targets, _ = fit.fit_workload(nscenes=2, seed=5, steps=4, dt=5e-4,
+                                         n_range=(1024, 2048), side_range=(32, 40))
+    scenes = [s.make_scene() for s in specs]
+    fb = FitBatch(scenes, targets, 4, 2, specs[0].dt)
+    fb.run()
+    r = specs[1].params.copy()
+    r.couple_band = 0.0
+    scenes[1].set_params(r)
+    with pytest.raises(cwa.CwaError) as e:
+        fb.run()
+    assert e.value.code == -4
+    fb.close()
+    for s in scenes:
+        s.close()

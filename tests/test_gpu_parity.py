@@ -97,6 +97,15 @@ def _grid_cases(cwa):
     pos = np.ones((64, 4), np.float32)                        # points on cell faces
     pos[:, :3] = (rng.integers(0, 40, (64, 3)) * p.cell_size).astype(np.float32)
     out.append(("faces", p, pos))
+    # 200 000 particles in four cells (50 000 each), slots interleaved and
+    # then in runs: the stable rank walks ~800 run records per cell
+    pos = np.ones((200_000, 4), np.float32)
+    cells = rng.integers(0, 4, 200_000)
+    cells[100_000:] = np.sort(cells[100_000:])
+    corner = np.array([[1, 1, 1], [1, 2, 1], [30, 7, 12], [45, 90, 45]], np.float32)
+    corner = np.minimum(corner, np.array(p.grid_dims, np.float32) - 1)
+    pos[:, :3] = (corner[cells] + rng.uniform(0.1, 0.9, (200_000, 3))) * p.cell_size
+    out.append(("clustered", p, pos.astype(np.float32)))
     return out
 
 
@@ -121,6 +130,61 @@ def test_grid_build_bit_exact_1m(gpu, cwa, oracle):
     r = oracle.grid_build(s.params, s.pos)
     for k in ("key", "count", "offset", "list"):
         assert np.array_equal(g[k], r[k]), k
+
+
+def _gpu_boxes(cwa, torch, dev, x, hw, gmin, cell, dims, capacity=None):
+    import ctypes as C
+    from coupledwateranimation_amd import _lib
+
+    n = x.shape[0]
+    nc = dims[0] * dims[1] * dims[2]
+    dx = torch.tensor(x, device=dev)
+    dhw = torch.tensor(hw, device=dev)
+    cnt = torch.empty(nc, dtype=torch.int32, device=dev)
+    off = torch.empty(nc + 1, dtype=torch.int32, device=dev)
+    cap = 8 * max(n, 1) if capacity is None else capacity
+    lst = torch.full((max(cap, 1),), -1, dtype=torch.int32, device=dev)
+    total = C.c_int64(-1)
+    rc = cwa.lib().cwa_grid_build_boxes(
+        dx.data_ptr() if n else None, dhw.data_ptr() if n else None, n,
+        (C.c_float * 3)(*gmin), (C.c_float * 3)(*cell), (C.c_int32 * 3)(*dims), cnt.data_ptr(),
+        off.data_ptr(), lst.data_ptr(), cap, C.byref(total), None)
+    return rc, total.value, {"count": cnt.cpu().numpy(), "offset": off.cpu().numpy(),
+                             "list": lst.cpu().numpy()[:max(total.value, 0)]}
+
+
+def test_box_grid_bit_exact(gpu, cwa, oracle):
+    """cwa_grid_build_boxes (UniformGridPbdGpu3D's ComputeCounts / scan /
+    InsertBox) against the oracle: boxes over 1, 2, 4 and 8 cells, boxes
+    clamped at the grid's edges and outside it, inverted and zero-size boxes,
+    a non-cubic grid with a cell size per axis, 200 000 random boxes; counts,
+    offsets and the list (ascending boxes per cell) bit-exact.  Too small a
+    list is refused with the pair total reported."""
+    import torch
+    from test_oracle_scan_grid import _box_cases
+
+    rng = np.random.default_rng(5)
+    cases = _box_cases()
+    n = 200_000
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = rng.uniform(-1.0, 1.0, (n, 3))
+    hw = np.zeros((n, 4), np.float32)
+    hw[:, :3] = rng.uniform(0.0, 0.1, (n, 3))
+    cases.append(("random_200k", x, hw, (-1.0, -1.0, -1.0), (2 / 40, 2 / 40, 2 / 40),
+                  (40, 40, 40)))
+    for name, x, hw, gmin, cell, dims in cases:
+        _say(f"box grid {name}: {x.shape[0]} boxes")
+        ref = oracle.grid_build_boxes(x, hw, gmin, cell, dims)
+        rc, total, g = _gpu_boxes(cwa, torch, gpu, x, hw, gmin, cell, dims,
+                                  capacity=max(ref["total"], 8 * x.shape[0]))
+        assert rc == 0 and total == ref["total"], (name, rc, total)
+        for k in ("count", "offset", "list"):
+            assert np.array_equal(g[k], ref[k]), (name, k)
+    name, x, hw, gmin, cell, dims = cases[1]
+    ref = oracle.grid_build_boxes(x, hw, gmin, cell, dims)
+    rc, total, _ = _gpu_boxes(cwa, torch, gpu, x, hw, gmin, cell, dims,
+                              capacity=ref["total"] - 1)
+    assert rc == -1 and total == ref["total"]
 
 
 def test_scene_grid_over_2m_cells(gpu, cwa, oracle):

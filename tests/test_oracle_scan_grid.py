@@ -123,3 +123,95 @@ def test_neighbor_sets_grid_equals_all_pairs(oracle, cwa, seed, ydiv):
     b = oracle.neighbor_counts(p, pos, all_pairs=True)
     assert a.tolist() == b.tolist()
     assert a.min() >= 1  # self included
+
+
+# ---------------------------------------------------------------- box grid
+def _box_cases():
+    """(name, centres, half-widths, grid_min, cell, dims)."""
+    rng = np.random.default_rng(21)
+    gmin, dims = (-1.0, -1.0, -1.0), (16, 16, 16)
+    cell = (2.0 / 16, 2.0 / 16, 2.0 / 16)
+    out = []
+    n = 4000
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = rng.uniform(-0.95, 0.95, (n, 3))
+    hw = np.zeros((n, 4), np.float32)
+    hw[:, :3] = rng.uniform(0.0, 0.5 * cell[0], (n, 3))    # the reference's <= 8 cells
+    out.append(("small", x, hw, gmin, cell, dims))
+    for span in (1, 2, 4, 8):                                # boxes over 1, 2, 4, 8 cells
+        m = 500
+        xs = np.zeros((m, 4), np.float32)
+        xs[:, :3] = rng.uniform(-0.9, 0.9, (m, 3))
+        hs = np.zeros((m, 4), np.float32)
+        hs[:, :3] = 0.5 * span * cell[0] * rng.uniform(0.3, 0.95, (m, 3))
+        out.append((f"span{span}", xs, hs, gmin, cell, dims))
+    m = 300                                                  # partly or fully outside
+    xs = np.zeros((m, 4), np.float32)
+    xs[:, :3] = rng.uniform(-1.6, 1.6, (m, 3))
+    hs = np.zeros((m, 4), np.float32)
+    hs[:, :3] = rng.uniform(0.0, 0.4, (m, 3))
+    out.append(("edges", xs, hs, gmin, cell, dims))
+    xs = np.zeros((6, 4), np.float32)                        # inverted / degenerate
+    xs[:, :3] = [[0.1, 0.2, 0.3], [0.0, 0.0, 0.0], [-0.5, 0.7, 0.1], [0.3, 0.3, 0.3],
+                 [0.9, -0.9, 0.5], [5.0, 5.0, 5.0]]
+    hs = np.zeros((6, 4), np.float32)
+    hs[:, :3] = [[-0.2, 0.1, 0.1], [0.0, 0.0, 0.0], [0.05, -0.3, 0.05], [0.3, 0.3, 0.3],
+                 [0.0, 0.0, 0.0], [0.1, 0.1, 0.1]]
+    out.append(("degenerate", xs, hs, gmin, cell, dims))
+    m = 2000                                                 # non-cubic grid, cells per axis
+    xs = np.zeros((m, 4), np.float32)
+    xs[:, :3] = rng.uniform([0, 0, 0], [3.0, 1.0, 2.0], (m, 3))
+    hs = np.zeros((m, 4), np.float32)
+    hs[:, :3] = rng.uniform(0.0, 0.15, (m, 3))
+    out.append(("aniso", xs, hs, (0.0, 0.0, 0.0), (0.125, 0.05, 0.1), (24, 20, 20)))
+    out.append(("empty", np.zeros((0, 4), np.float32), np.zeros((0, 4), np.float32), gmin,
+                cell, dims))
+    return out
+
+
+def _box_brute_force(x, hw, gmin, cell, dims):
+    """Per cell, the boxes whose closed interval [x - hw, x + hw] (corners
+    rounded to fp32) meets the cell's half-open interval on every axis; the
+    edge cells reach to infinity (points outside the grid clamp into them).
+    Ascending box order within each cell."""
+    n = x.shape[0]
+    a = (x[:, :3] - hw[:, :3]).astype(np.float64)
+    b = (x[:, :3] + hw[:, :3]).astype(np.float64)
+    per_axis = []
+    for ax in range(3):
+        i = np.arange(dims[ax])
+        lo = np.float64(np.float32(gmin[ax])) + i * np.float64(np.float32(cell[ax]))
+        hi = lo + np.float64(np.float32(cell[ax]))
+        lo[0], hi[-1] = -np.inf, np.inf
+        per_axis.append((a[:, ax:ax + 1] < hi[None, :]) & (b[:, ax:ax + 1] >= lo[None, :]))
+    cells = {}
+    for k in range(n):
+        for ix in np.nonzero(per_axis[0][k])[0]:
+            for iy in np.nonzero(per_axis[1][k])[0]:
+                for iz in np.nonzero(per_axis[2][k])[0]:
+                    c = (ix * dims[2] + iz) * dims[1] + iy
+                    cells.setdefault(c, []).append(k)
+    nc = dims[0] * dims[1] * dims[2]
+    count = np.zeros(nc, np.int64)
+    lst = []
+    for c in range(nc):
+        count[c] = len(cells.get(c, []))
+        lst += cells.get(c, [])
+    return count, np.array(lst, np.int64)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _box_cases()])
+def test_box_grid_against_brute_force(oracle, case):
+    """The oracle's box grid (UniformGridPbdGpu3D: ComputeCounts / InsertBox
+    over ComputeCellIndex(x -+ hw)) equals a brute-force overlap of every box
+    with every cell, and lists the boxes of a cell in ascending order."""
+    name, x, hw, gmin, cell, dims = next(c for c in _box_cases() if c[0] == case)
+    g = oracle.grid_build_boxes(x, hw, gmin, cell, dims)
+    count, lst = _box_brute_force(x, hw, gmin, cell, dims)
+    assert g["count"].tolist() == count.tolist()
+    assert g["offset"].tolist() == np.concatenate([[0], np.cumsum(count)]).tolist()
+    assert g["total"] == len(lst) and g["list"].tolist() == lst.tolist()
+    if name.startswith("span"):
+        span = int(name[4:])
+        per_box = np.bincount(lst, minlength=x.shape[0])
+        assert per_box.max() <= (span + 1) ** 3 and per_box.min() >= 1
