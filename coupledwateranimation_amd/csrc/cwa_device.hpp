@@ -110,46 +110,60 @@ __device__ __forceinline__ void column_range(const DevParams& P, const int* __re
 
 // ---------------------------------------------------------------------------
 // Neighbour words: the exact neighbour set of a particle as recorded by the
-// density pass, for the passes that visit it again (force, adjoint).
-// Per candidate column c (0..8) of particle i:
-//   nbw[c*n + i]     = (bits of candidates 0..31, b | flags)
-//   nbw[(9+c)*n + i] = (bits of candidates 32..63, b + 32)   only if kNbHi
-// with b the column's first candidate slot and bit k set when slot base + k
-// is a neighbour other than i itself.  kNbHi marks a column with a
-// neighbour among candidates 32..63; kNbTail (on column 0's entry) marks a
-// particle with a column longer than 64 candidates, whose neighbours the
-// consumers re-search with the exact distance test instead.
+// density pass, for the passes that visit it again (force, adjoint).  For
+// each candidate column in order, each 32-candidate chunk of the column that
+// holds a neighbour of particle i (other than i itself) gives one word
+// (bits, base): bit k set when slot base + k is a neighbour.  The words of
+// particle i are packed, in candidate order:
+//   nbw[k*n + i]                 word k, k < count
+//   nb_count(nbw, n)[i]          count, or -1: the particle has more words
+//                                than kNbEntries or a column longer than
+//                                kNbMaxColumn candidates, and its consumers
+//                                re-search its neighbours with the exact
+//                                distance test instead
+// in one buffer of nb_buffer_uint2(n) uint2 (the counts follow the words).
+// Only a particle's non-empty words are written and read back.
 // ---------------------------------------------------------------------------
 constexpr int kNbEntries = 18;
-constexpr uint32_t kNbHi = 1u << 31;
-constexpr uint32_t kNbTail = 1u << 30;
-constexpr uint32_t kNbSlot = kNbTail - 1u;
+constexpr int kNbMaxColumn = 64;
 
-// The non-zero words of one particle (at most kNbEntries), staged per
-// thread in LDS (slot t of the thread at s[t * stride]) by stage_nb_words,
-// then walked in order by NbCursor: one flat loop over all columns, so a
-// wave runs as many iterations as its busiest lane has neighbours rather
-// than the sum over columns of each column's busiest lane.  Room for every
-// word also caps a 256-thread block at 36 KB of LDS, four blocks per CU,
-// which measured faster for the force pass than six (fewer waves gathering
-// neighbour records through one L1).
+__host__ __device__ constexpr size_t nb_buffer_uint2(size_t n) {
+    return (size_t)kNbEntries * n + (n + 1) / 2;
+}
+__device__ __forceinline__ int* nb_count(uint2* nbw, int n) {
+    return reinterpret_cast<int*>(nbw + (size_t)kNbEntries * n);
+}
+__device__ __forceinline__ const int* nb_count(const uint2* nbw, int n) {
+    return reinterpret_cast<const int*>(nbw + (size_t)kNbEntries * n);
+}
+
+// Appends the word (bits, base) of particle i when bits is non-zero.
+__device__ __forceinline__ void nb_put(uint2* __restrict__ nbw, int n, int i, int& w,
+                                       uint32_t bits, uint32_t base) {
+    if (!bits) return;
+    if (w < kNbEntries) nbw[(size_t)w * n + i] = make_uint2(bits, base);
+    ++w;
+}
+__device__ __forceinline__ void nb_finish(uint2* __restrict__ nbw, int n, int i, int w,
+                                          bool long_column) {
+    nb_count(nbw, n)[i] = (long_column || w > kNbEntries) ? -1 : w;
+}
+
+// The words of one particle (at most kNbEntries), staged per thread in LDS
+// (slot t of the thread at s[t * stride]) by stage_nb_words, then walked in
+// order by NbCursor: one flat loop over all columns, so a wave runs as many
+// iterations as its busiest lane has neighbours rather than the sum over
+// columns of each column's busiest lane.  Room for every word also caps a
+// 256-thread block at 36 KB of LDS, four blocks per CU, which measured
+// faster for the force pass than six (fewer waves gathering neighbour
+// records through one L1).
 constexpr int kNbStage = kNbEntries;
 
-// Number of words staged, or -1 when the particle must be re-searched
-// (a column past 64 candidates).
+// Number of words staged, or -1 when the particle must be re-searched.
 __device__ __forceinline__ int stage_nb_words(const uint2* __restrict__ nbw, int n, int i,
                                               uint2* s, int stride) {
-    uint2 e[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) e[c] = nbw[(size_t)c * n + i];
-    if (e[0].y & kNbTail) return -1;
-    int nw = 0;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-        const uint32_t b = e[c].y & kNbSlot;
-        if (e[c].x) s[(nw++) * stride] = make_uint2(e[c].x, b);
-        if (e[c].y & kNbHi) s[(nw++) * stride] = nbw[(size_t)(9 + c) * n + i];
-    }
+    const int nw = nb_count(nbw, n)[i];
+    for (int k = 0; k < nw; ++k) s[k * stride] = nbw[(size_t)k * n + i];
     return nw;
 }
 

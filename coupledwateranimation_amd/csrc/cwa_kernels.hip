@@ -482,8 +482,8 @@ __device__ __forceinline__ void density_finish(const DevParams& P, int i, float4
 // ---------------------------------------------------------------------------
 // Density / pressure (rho_pres_comp.glsl:49-81) + particle flux term.
 // Also records the exact neighbour set of every particle as neighbour words
-// (nbw: per candidate column the bits of its first 64 candidates, the
-// particle itself left out; layout and flags at kNbEntries).
+// (the packed non-empty words of each candidate column's 32-candidate
+// chunks, the particle itself left out; layout at kNbEntries).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
@@ -504,8 +504,8 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
-    uint2 e0 = make_uint2(0u, 0u);  // column 0's entry, written last
-    bool tail = false;
+    int w = 0;          // words written
+    bool tail = false;  // a column longer than kNbMaxColumn candidates
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
@@ -526,17 +526,15 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
             for (k = 32; k < l2; ++k) density_candidate(pi, q[k], T, acc, hi, k - 32);
             uint32_t none = 0;
             for (k = 64; k < len; ++k) density_candidate(pi, q[k], T, acc, none, 0);
-            tail |= len > 64;
+            tail |= len > kNbMaxColumn;
         }
         const uint32_t self = (uint32_t)(i - b);  // i is a candidate of one column
         if (self < 32u) lo &= ~(1u << self);
         else if (self < 64u) hi &= ~(1u << (self - 32u));
-        const uint2 ent = make_uint2(lo, (uint32_t)b | (hi ? kNbHi : 0u));
-        if (c == 0) e0 = ent;
-        else nbw[(size_t)c * n + i] = ent;
-        if (hi) nbw[(size_t)(9 + c) * n + i] = make_uint2(hi, (uint32_t)b + 32u);
+        nb_put(nbw, n, i, w, lo, (uint32_t)b);
+        nb_put(nbw, n, i, w, hi, (uint32_t)b + 32u);
     }
-    nbw[i] = make_uint2(e0.x, e0.y | (tail ? kNbTail : 0u));
+    nb_finish(nbw, n, i, w, tail);
     density_finish(P, i, pi, vi, acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
@@ -563,8 +561,8 @@ __global__ void __launch_bounds__(256) k_density_split(
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
-    uint2 e0 = make_uint2(0u, 0u);  // column 0's entry, written last
-    bool tail = false;
+    int w = 0;          // words written (by the group's first lane)
+    bool tail = false;  // a column longer than kNbMaxColumn candidates
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         int b, e;
@@ -582,19 +580,19 @@ __global__ void __launch_bounds__(256) k_density_split(
             lo |= __shfl_xor(lo, o, 64);
             hi |= __shfl_xor(hi, o, 64);
         }
-        tail |= len > 64;
+        tail |= len > kNbMaxColumn;
         const uint32_t self = (uint32_t)(i - b);  // i is a candidate of one column
         if (self < 32u) lo &= ~(1u << self);
         else if (self < 64u) hi &= ~(1u << (self - 32u));
-        const uint2 ent = make_uint2(lo, (uint32_t)b | (hi ? kNbHi : 0u));
-        if (c == 0) e0 = ent;
-        else if (sub == 0) nbw[(size_t)c * n + i] = ent;
-        if (hi && sub == 0) nbw[(size_t)(9 + c) * n + i] = make_uint2(hi, (uint32_t)b + 32u);
+        if (sub == 0) {
+            nb_put(nbw, n, i, w, lo, (uint32_t)b);
+            nb_put(nbw, n, i, w, hi, (uint32_t)b + 32u);
+        }
     }
 #pragma unroll
     for (int o = 1; o < kDensitySplit; o <<= 1) acc += __shfl_xor(acc, o, 64);
     if (sub != 0) return;
-    nbw[i] = make_uint2(e0.x, e0.y | (tail ? kNbTail : 0u));
+    nb_finish(nbw, n, i, w, tail);
     density_finish(P, i, pi, vel[i], acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
@@ -825,15 +823,11 @@ __global__ void __launch_bounds__(256) k_neighbour_counts(const DevParams* __res
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int k = 1;  // the particle itself
-    const uint2 e0 = nbw[i];
-    if (e0.y & kNbTail) {
+    const int nw = nb_count(nbw, n)[i];
+    if (nw < 0) {
         for_each_candidate_neighbour(P, offset, pos, i, pos[i], [&](int) { ++k; });
     } else {
-        for (int c = 0; c < 9; ++c) {
-            const uint2 e = c == 0 ? e0 : nbw[(size_t)c * n + i];
-            k += __popc(e.x);
-            if (e.y & kNbHi) k += __popc(nbw[(size_t)(9 + c) * n + i].x);
-        }
+        for (int w = 0; w < nw; ++w) k += __popc(nbw[(size_t)w * n + i].x);
     }
     out[i] = k;
 }

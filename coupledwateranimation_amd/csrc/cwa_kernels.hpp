@@ -48,7 +48,8 @@ hipError_t launch_sum_i64(const int* a, int n, unsigned long long* out, hipStrea
 hipError_t launch_box_pairs(const float4* x, const float4* hw, int n, const float gmin[3],
                             const float cell[3], const int dims[3], const int* first,
                             int* pkey, int* pbox, hipStream_t st);
-// nbw (kNbEntries*n uint2, neighbour words) and nrec (2*n float4) are written by the density pass
+// nbw (nb_buffer_uint2(n) uint2: neighbour words and counts) and nrec (2*n
+// float4) are written by the density pass
 // and read by the force pass.  rho_raw (density before the wave term and the
 // rest-density clamp) and fpre (force before the foam branch) are optional
 // outputs for the adjoint tape (null: not written).

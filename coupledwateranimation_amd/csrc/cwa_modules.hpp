@@ -213,7 +213,7 @@ class SphParticles3D {
 
    private:
     int n_ = 0;
-    DeviceBuffer<uint2> nbw_;  // kNbEntries x n neighbour words of the last step
+    DeviceBuffer<uint2> nbw_;  // neighbour words and counts of the last step
     DeviceBuffer<float4> nrec_;   // 2 x n neighbour records of the last step
     DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
     DeviceBuffer<int> id_[2];

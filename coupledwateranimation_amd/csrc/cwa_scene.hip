@@ -191,7 +191,7 @@ void SphParticles3D::Allocate(int n) {
     rho_w_.resize(m);
     flux_.resize(m);
     pw_.resize(m);
-    nbw_.resize((size_t)kNbEntries * m);
+    nbw_.resize(nb_buffer_uint2(m));
     nrec_.resize(2 * m);
     A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
     B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};

@@ -181,7 +181,7 @@ void Tape::alloc_step_tape(StepTape& t) {
     t.fpre.resize(n);
     t.list.resize(n);
     t.offset.resize((size_t)ncell_ + 1);
-    t.nbw.resize((size_t)kNbEntries * n);
+    t.nbw.resize(nb_buffer_uint2(n));
     t.rho.resize(n);
     t.rho_raw.resize(n);
     t.pw.resize(n);
