@@ -137,17 +137,36 @@ __device__ __forceinline__ const int* nb_count(const uint2* nbw, int n) {
     return reinterpret_cast<const int*>(nbw + (size_t)kNbEntries * n);
 }
 
-// Appends the word (bits, base) of particle i when bits is non-zero.
-__device__ __forceinline__ void nb_put(uint2* __restrict__ nbw, int n, int i, int& w,
-                                       uint32_t bits, uint32_t base) {
-    if (!bits) return;
-    if (w < kNbEntries) nbw[(size_t)w * n + i] = make_uint2(bits, base);
-    ++w;
-}
-__device__ __forceinline__ void nb_finish(uint2* __restrict__ nbw, int n, int i, int w,
-                                          bool long_column) {
-    nb_count(nbw, n)[i] = (long_column || w > kNbEntries) ? -1 : w;
-}
+// Writer of one particle's words.  A lane's k-th word belongs in plane k,
+// and lanes reach plane k at different columns, so a store issued per word
+// would scatter each instruction over several planes.  The first kNbHeld
+// words wait in LDS (slot k of the thread at held[k * stride]) and are
+// stored plane by plane at the end, each store one contiguous run of the
+// wave's particles; later words (rare) are stored as they come.
+constexpr int kNbHeld = 8;
+
+struct NbWriter {
+    uint2* held;  // kNbHeld LDS slots of this thread, stride `stride`
+    int stride;
+    int w = 0;    // words so far
+    __device__ __forceinline__ NbWriter(uint2* h, int st) : held(h), stride(st) {}
+    // Appends the word (bits, base) of particle i when bits is non-zero.
+    __device__ __forceinline__ void put(uint2* __restrict__ nbw, int n, int i, uint32_t bits,
+                                        uint32_t base) {
+        if (!bits) return;
+        const uint2 v = make_uint2(bits, base);
+        if (w < kNbHeld) held[w * stride] = v;
+        else if (w < kNbEntries) nbw[(size_t)w * n + i] = v;
+        ++w;
+    }
+    __device__ __forceinline__ void finish(uint2* __restrict__ nbw, int n, int i,
+                                           bool long_column) {
+#pragma unroll
+        for (int k = 0; k < kNbHeld; ++k)
+            if (k < w) nbw[(size_t)k * n + i] = held[k * stride];
+        nb_count(nbw, n)[i] = (long_column || w > kNbEntries) ? -1 : w;
+    }
+};
 
 // The words of one particle (at most kNbEntries), staged per thread in LDS
 // (slot t of the thread at s[t * stride]) by stage_nb_words, then walked in

@@ -504,7 +504,8 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
-    int w = 0;          // words written
+    __shared__ uint2 s_held[kNbHeld][256];
+    NbWriter wr(&s_held[0][threadIdx.x], 256);
     bool tail = false;  // a column longer than kNbMaxColumn candidates
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
@@ -531,10 +532,10 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
         const uint32_t self = (uint32_t)(i - b);  // i is a candidate of one column
         if (self < 32u) lo &= ~(1u << self);
         else if (self < 64u) hi &= ~(1u << (self - 32u));
-        nb_put(nbw, n, i, w, lo, (uint32_t)b);
-        nb_put(nbw, n, i, w, hi, (uint32_t)b + 32u);
+        wr.put(nbw, n, i, lo, (uint32_t)b);
+        wr.put(nbw, n, i, hi, (uint32_t)b + 32u);
     }
-    nb_finish(nbw, n, i, w, tail);
+    wr.finish(nbw, n, i, tail);
     density_finish(P, i, pi, vi, acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
@@ -561,7 +562,9 @@ __global__ void __launch_bounds__(256) k_density_split(
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
     float acc = 0.0f;
-    int w = 0;          // words written (by the group's first lane)
+    // words written by the group's first lane
+    __shared__ uint2 s_held[kNbHeld][256 / kDensitySplit];
+    NbWriter wr(&s_held[0][threadIdx.x / kDensitySplit], 256 / kDensitySplit);
     bool tail = false;  // a column longer than kNbMaxColumn candidates
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
@@ -585,14 +588,14 @@ __global__ void __launch_bounds__(256) k_density_split(
         if (self < 32u) lo &= ~(1u << self);
         else if (self < 64u) hi &= ~(1u << (self - 32u));
         if (sub == 0) {
-            nb_put(nbw, n, i, w, lo, (uint32_t)b);
-            nb_put(nbw, n, i, w, hi, (uint32_t)b + 32u);
+            wr.put(nbw, n, i, lo, (uint32_t)b);
+            wr.put(nbw, n, i, hi, (uint32_t)b + 32u);
         }
     }
 #pragma unroll
     for (int o = 1; o < kDensitySplit; o <<= 1) acc += __shfl_xor(acc, o, 64);
     if (sub != 0) return;
-    nb_finish(nbw, n, i, w, tail);
+    wr.finish(nbw, n, i, tail);
     density_finish(P, i, pi, vel[i], acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
