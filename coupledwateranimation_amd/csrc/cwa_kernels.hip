@@ -253,34 +253,32 @@ hipError_t launch_scan(const int* in, int* out, int n, int* ws, hipStream_t st,
 // ugrid_particles_cs.glsl:90-117 (whose order within a cell is not
 // deterministic) with the UniformGrid2D::Build order (UniformGrid2D.cpp:
 // 75-91: items ascending within each cell):
-//   count    one atomicAdd per run of equal keys in a wave (lane order), by
-//            the run's first lane, which keeps the run's provisional base;
-//            the other lanes keep minus their distance to the run head
+//   count    one atomicAdd per run of equal keys in a wave (lane order),
+//            issued by the run's first lane, its result unused (the wave
+//            does not wait for it)
 //   scan     exclusive scan of the counts -> offsets
-//   scatter  each run head writes (first item, length) at its provisional
-//            place in the cell's segment; the runs tile the segment
+//   scatter  each run head takes a provisional place in its cell's segment
+//            from a fill counter and writes (first item, length) there; the
+//            runs tile the segment
 //   reorder  the run head walks its cell's run records (one per run, not one
 //            per item) and counts the items of the runs that start before
 //            its own; the lanes of the run take that rank plus their
-//            distance to the head.  The particle state moves to its sorted
-//            slot in the same pass.
+//            distance to the head.  The particle positions and ids move to
+//            their sorted slots in the same pass, which also clears the
+//            fill counters for the next build.
 // The wave layout (blocks of 256 over the same item range) is identical in
-// the three passes, so every pass sees the same runs.
+// the three item passes, so every pass sees the same runs.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void count_runs(int c, bool valid, int* __restrict__ rec,
-                                           int* __restrict__ count, int i) {
+__device__ __forceinline__ void count_runs(int c, bool valid, int* __restrict__ count) {
     const int lane = threadIdx.x & 63;
     const RunInfo r = wave_runs(c);
-    int base = 0;
-    if (r.head && valid) base = atomicAdd(&count[c], r.run_end - lane);
-    if (valid) rec[i] = r.head ? base : -(lane - r.my_head);
+    if (r.head && valid) atomicAdd(&count[c], r.run_end - lane);
 }
 
 // Cell hash + count of particle slots (ComputeCellIndex / Index / ComputeCounts,
 // ugrid_particles_cs.glsl:90-108, with the y-fastest linearisation).
 __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ pos, int n,
                                                     GridDesc g, int* __restrict__ key,
-                                                    int* __restrict__ rec,
                                                     int* __restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
@@ -293,42 +291,47 @@ __global__ void __launch_bounds__(256) k_hash_count(const float4* __restrict__ p
         c = lin_cell(g.dims, ix, iy, iz);
         key[i] = c;
     }
-    count_runs(c, valid, rec, count, i);
+    count_runs(c, valid, count);
 }
 
 // Count of items whose keys are already known.
 __global__ void __launch_bounds__(256) k_key_count(const int* __restrict__ key, int n,
-                                                   int* __restrict__ rec,
                                                    int* __restrict__ count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
-    count_runs(valid ? key[i] : -1, valid, rec, count, i);
+    count_runs(valid ? key[i] : -1, valid, count);
 }
 
-// Run records: (first item, length) at the run's provisional place.
+// Run records: (first item, length) at a provisional place in the cell's
+// segment (fill: zero on entry).
 __global__ void __launch_bounds__(256) k_scatter_runs(const int* __restrict__ key,
-                                                      const int* __restrict__ rec,
-                                                      const int* __restrict__ offset, int n,
+                                                      const int* __restrict__ offset,
+                                                      int* __restrict__ fill, int n,
                                                       int2* __restrict__ runs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const int c = valid ? key[i] : -1;
     const RunInfo r = wave_runs(c);
-    if (valid && r.head) runs[offset[c] + rec[i]] = make_int2(i, r.run_end - lane);
+    if (valid && r.head) {
+        const int len = r.run_end - lane;
+        runs[offset[c] + atomicAdd(&fill[c], len)] = make_int2(i, len);
+    }
 }
 
 // Stable destination of every item; list[d] = item_of[item] (or the item
-// itself), and the particle state moves when pos_out is set.
+// itself), and the particle positions and ids move to their sorted slots
+// when pos_out is set (the passes that follow read velocities and forces in
+// their pre-sort slots through list).  Threads below nfill clear fill.
 __global__ void __launch_bounds__(256) k_stable_reorder(
-    const int* __restrict__ key, const int* __restrict__ rec, const int* __restrict__ offset,
+    const int* __restrict__ key, const int* __restrict__ offset,
     const int2* __restrict__ runs, int n, const int* __restrict__ item_of,
     int* __restrict__ list, const float4* __restrict__ pos_in,
-    const float4* __restrict__ vel_in, const float4* __restrict__ force_in,
-    const int* __restrict__ id_in, float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-    float4* __restrict__ force_out, int* __restrict__ id_out) {
+    const int* __restrict__ id_in, float4* __restrict__ pos_out, int* __restrict__ id_out,
+    int* __restrict__ fill, int nfill) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    if (s < nfill) fill[s] = 0;
     const bool valid = s < n;
     const int c = valid ? key[s] : -1;
     const RunInfo r = wave_runs(c);
@@ -348,8 +351,6 @@ __global__ void __launch_bounds__(256) k_stable_reorder(
     list[d] = item_of ? item_of[s] : s;
     if (pos_out) {
         pos_out[d] = pos_in[s];
-        vel_out[d] = vel_in[s];
-        force_out[d] = force_in[s];
         id_out[d] = id_in[s];
     }
 }
@@ -463,7 +464,7 @@ __device__ __forceinline__ void density_finish(const DevParams& P, int i, float4
     rho = fmaxf(P.rho0, rho);
     const float irho = 1.0f / rho;
     rho_w[i] = rho;
-    pw[i] = make_float2(pressure, irho);
+    if (pw) pw[i] = make_float2(pressure, irho);  // taped for the adjoint
     // the 32-byte record the force pass reads for this particle as a neighbour
     nrec[2 * (size_t)i] = make_float4(pi.x, pi.y, pi.z, pressure);
     nrec[2 * (size_t)i + 1] = make_float4(vi.x, vi.y, vi.z, irho);
@@ -487,6 +488,7 @@ __device__ __forceinline__ void density_finish(const DevParams& P, int i, float4
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel,
+                                                 const int* __restrict__ list,
                                                  const int* __restrict__ offset, int n,
                                                  const float* __restrict__ H0,
                                                  float* __restrict__ rho_w,
@@ -494,12 +496,16 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
                                                  float* __restrict__ flux,
                                                  uint2* __restrict__ nbw,
                                                  float4* __restrict__ nrec,
-                                                 float* __restrict__ rho_raw) {
+                                                 float* __restrict__ rho_raw,
+                                                 float4* __restrict__ vel_out,
+                                                 float* __restrict__ vpad) {
     const DevParams& P = *Pd;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pos[i];
-    const float4 vi = vel[i];
+    const float4 vi = vel[list[i]];  // the state's velocity, in its pre-sort slot
+    if (vel_out) vel_out[i] = vi;
+    vpad[i] = vi.w;
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
@@ -549,15 +555,17 @@ constexpr int kDensitySplit = 4;
 constexpr int kDensitySplitMaxN = 1 << 18;
 __global__ void __launch_bounds__(256) k_density_split(
     const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
-    const float4* __restrict__ vel, const int* __restrict__ offset, int n,
-    const float* __restrict__ H0, float* __restrict__ rho_w, float2* __restrict__ pw,
-    float* __restrict__ flux, uint2* __restrict__ nbw, float4* __restrict__ nrec,
-    float* __restrict__ rho_raw) {
+    const float4* __restrict__ vel, const int* __restrict__ list,
+    const int* __restrict__ offset, int n, const float* __restrict__ H0,
+    float* __restrict__ rho_w, float2* __restrict__ pw, float* __restrict__ flux,
+    uint2* __restrict__ nbw, float4* __restrict__ nrec, float* __restrict__ rho_raw,
+    float4* __restrict__ vel_out, float* __restrict__ vpad) {
     const DevParams& P = *Pd;
     const int sub = threadIdx.x & (kDensitySplit - 1);
     const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kDensitySplit);
     if (i >= n) return;  // whole groups: the lanes of a group share i
     const float4 pi = pos[i];
+    const float4 vi = vel[list[i]];  // the state's velocity, in its pre-sort slot
     const int ix = cell_axis(pi.x, P.gmin[0], P.cell, P.dims[0]);
     const int iz = cell_axis(pi.z, P.gmin[2], P.cell, P.dims[2]);
     const float T = P.r2_lt;
@@ -596,7 +604,9 @@ __global__ void __launch_bounds__(256) k_density_split(
     for (int o = 1; o < kDensitySplit; o <<= 1) acc += __shfl_xor(acc, o, 64);
     if (sub != 0) return;
     wr.finish(nbw, n, i, tail);
-    density_finish(P, i, pi, vel[i], acc, H0, rho_w, pw, flux, nrec, rho_raw);
+    if (vel_out) vel_out[i] = vi;
+    vpad[i] = vi.w;
+    density_finish(P, i, pi, vi, acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
 // Pressure + viscosity contribution of one neighbour (force_comp.glsl:85-86).
@@ -659,21 +669,27 @@ __device__ __forceinline__ void column_flux_sum(const int* __restrict__ offset,
 // ---------------------------------------------------------------------------
 // Forces (force_comp.glsl:62-139) fused with symplectic Euler + boundary
 // (integrate_comp.glsl:56-92, 135-178).  Visits only the neighbours the
-// density pass recorded (its neighbour words, in candidate order), reads
-// the sorted copy B and writes the state A in the same slot order.
+// density pass recorded (its neighbour words, in candidate order).  Reads
+// the particle's own record from the density pass (position, pressure,
+// velocity), its id from the sorted copy and its previous force from the
+// state in its pre-sort slot (list[i]); writes the next state in the sorted
+// slot order.  When `next` is set it also hashes the new position into the
+// next step's grid (cell key, run record and count, ComputeCounts of
+// ugrid_particles_cs.glsl:90-95), so that step needs no hash pass of its own.
 // Blocks nfb.. of the grid sum the particle flux per column instead (one
 // wave per column; they read only the density pass's output and run in
 // the force blocks' tail).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_force_integrate(
-    const DevParams* __restrict__ Pd, float dt, const float4* __restrict__ pos, const float4* __restrict__ vel,
-    const float4* __restrict__ force, const int* __restrict__ id, const int* __restrict__ offset,
-    int n, const float* __restrict__ rho_w, const float2* __restrict__ pw,
+    const DevParams* __restrict__ Pd, float dt, const float4* __restrict__ bpos,
+    const int* __restrict__ bid, const float* __restrict__ vpad,
+    const float4* __restrict__ force, const int* __restrict__ list,
+    const int* __restrict__ offset, int n, const float* __restrict__ rho_w,
     const uint2* __restrict__ nbw, const float4* __restrict__ nrec,
     const float* __restrict__ H0, float4* __restrict__ pos_o,
     float4* __restrict__ vel_o, float4* __restrict__ force_o, int* __restrict__ id_o,
     float* __restrict__ rho_o, float* __restrict__ prs_o, float4* __restrict__ fpre, int nfb,
-    const float* __restrict__ pflux, float* __restrict__ colflux) {
+    const float* __restrict__ pflux, float* __restrict__ colflux, NextKeys next) {
     const DevParams& P = *Pd;
     if ((int)blockIdx.x >= nfb) {
         column_flux_sum(offset, pflux, P.dims[0], P.dims[1], P.dims[2], colflux,
@@ -682,132 +698,148 @@ __global__ void __launch_bounds__(256) k_force_integrate(
         return;
     }
     const int i = xcd_block(blockIdx.x, nfb) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    const float p_i = pw[i].x;
-    const float h = P.h;
-    const float m_spiky_half = P.m * P.spiky * 0.5f;
-    const float m_lap = P.m * P.lap;
-    PairAcc a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const bool valid = i < n;
+    int cell = -1;
     __shared__ uint2 s_nb[kNbStage][256];
-    uint2* st = &s_nb[0][threadIdx.x];
-    const int nw = stage_nb_words(nbw, n, i, st, 256);
-    if (nw >= 0) {
-        // kForceBatch neighbours per iteration: every record is loaded
-        // before any pair is evaluated (slots past the end of the list hold
-        // a zero record and are evaluated with zero weight); pairs are
-        // accumulated in order.  At four blocks per CU, 6 and 7 in flight
-        // measured fastest (MI355X, 1M-particle dam break: 4, 5, 6, 7, 8 ->
-        // 107, 105, 105, 104, 117 us).
-        constexpr int kForceBatch = 6;
-        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-        NbCursor nb(st, 256, nw);
-        for (;;) {
-            int j[kForceBatch];
-            j[0] = nb.next();
-            if (j[0] < 0) break;
+    if (valid) {
+        const float4 own_a = nrec[2 * (size_t)i];      // (x, y, z, p)
+        const float4 own_b = nrec[2 * (size_t)i + 1];  // (vx, vy, vz, 1/rho)
+        const int src = list[i];                       // the particle's slot in the state
+        // the previous force is used only after the neighbour loop; issued
+        // here, its load is in flight while the loop runs
+        float4 fp = force[src];
+        const float4 pi = make_float4(own_a.x, own_a.y, own_a.z, bpos[i].w);
+        const float4 vi = make_float4(own_b.x, own_b.y, own_b.z, vpad[i]);
+        const float p_i = own_a.w;
+        const float h = P.h;
+        const float m_spiky_half = P.m * P.spiky * 0.5f;
+        const float m_lap = P.m * P.lap;
+        PairAcc a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        uint2* st = &s_nb[0][threadIdx.x];
+        const int nw = stage_nb_words(nbw, n, i, st, 256);
+        if (nw >= 0) {
+            // kForceBatch neighbours per iteration: every record is loaded
+            // before any pair is evaluated (slots past the end of the list hold
+            // a zero record and are evaluated with zero weight); pairs are
+            // accumulated in order.  At four blocks per CU, 6 and 7 in flight
+            // measured fastest (MI355X, 1M-particle dam break: 4, 5, 6, 7, 8 ->
+            // 107, 105, 105, 104, 117 us).
+            constexpr int kForceBatch = 6;
+            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            NbCursor nb(st, 256, nw);
+            for (;;) {
+                int j[kForceBatch];
+                j[0] = nb.next();
+                if (j[0] < 0) break;
 #pragma unroll
-            for (int u = 1; u < kForceBatch; ++u) j[u] = nb.next();
-            Nbr r[kForceBatch];
+                for (int u = 1; u < kForceBatch; ++u) j[u] = nb.next();
+                Nbr r[kForceBatch];
 #pragma unroll
-            for (int u = 0; u < kForceBatch; ++u) {
-                r[u] = Nbr{zero, zero};
-                if (j[u] >= 0) r[u] = load_nbr(nrec, j[u]);
+                for (int u = 0; u < kForceBatch; ++u) {
+                    r[u] = Nbr{zero, zero};
+                    if (j[u] >= 0) r[u] = load_nbr(nrec, j[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kForceBatch; ++u) {
+                    const bool live = j[u] >= 0;
+                    force_pair(h, pi, vi, p_i, live ? m_spiky_half : 0.0f, live ? m_lap : 0.0f, r[u],
+                               a);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < kForceBatch; ++u) {
-                const bool live = j[u] >= 0;
-                force_pair(h, pi, vi, p_i, live ? m_spiky_half : 0.0f, live ? m_lap : 0.0f, r[u],
-                           a);
-            }
+        } else {
+            for_each_candidate_neighbour(P, offset, bpos, i, pi, [&](int j) {
+                force_pair(h, pi, vi, p_i, m_spiky_half, m_lap, load_nbr(nrec, j), a);
+            });
         }
-    } else {
-        for_each_candidate_neighbour(P, offset, pos, i, pi, [&](int j) {
-            force_pair(h, pi, vi, p_i, m_spiky_half, m_lap, load_nbr(nrec, j), a);
-        });
-    }
-    float vfx = a.vfx, vfy = a.vfy, vfz = a.vfz;
-    float4 fp = force[i];
-    if (pi.y > P.crest) {
-        fp.x /= P.brk_mass;
-        fp.y /= P.brk_mass;
-        fp.z /= P.brk_mass;
-        fp.w /= P.brk_mass;
-        vfx *= P.brk_visc;
-        vfy *= P.brk_visc;
-        vfz *= P.brk_visc;
-    }
-    vfx *= P.visc;
-    vfy *= P.visc;
-    vfz *= P.visc;
-    const float u = (pi.x - P.org[0]) * P.uvs[0];
-    const float v = (pi.z - P.org[1]) * P.uvs[1];
-    const float height = tex_sample(H0, P.W, P.H, u, v);
-    float tqx = (pi.y * fp.z - pi.z * fp.y) * P.torque;
-    float tqy = (pi.z * fp.x - pi.x * fp.z) * P.torque;
-    float tqz = (pi.x * fp.y - pi.y * fp.x) * P.torque;
-    float drx = 0.f, dry = 0.f, drz = 0.f, wfx = 0.f, wfy = 0.f, wfz = 0.f;
-    if (coupled(P, pi.y, height)) {
-        // WaveVelocity (force_comp.glsl:117-128)
-        const float hx = tex_sample(H0, P.W, P.H, u + P.du, v);
-        const float hy = tex_sample(H0, P.W, P.H, u, v + P.du);
-        const float wvx = (hx - height) / dt, wvy = (hy - height) / dt, wvz = (hx - hy) / P.du;
-        drx = -P.drag * (vi.x - wvx);
-        dry = -P.drag * (vi.y - wvy);
-        drz = -P.drag * (vi.z - wvz);
-        // WaveNormal (force_comp.glsl:131-139) = (H(u+1)-H, H(v+1)-H, -1)
-        const float nx = tex_sample(H0, P.W, P.H, u + 1.0f, v) - height;
-        const float ny = tex_sample(H0, P.W, P.H, u, v + 1.0f) - height;
-        wfx = -height * nx * P.wforce;
-        wfy = -height * ny * P.wforce;
-        wfz = -height * -1.0f * P.wforce;
-    }
-    float rho = rho_w[i];
-    float prs = p_i;
-    float Fx = a.pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
-    float Fy = a.pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
-    float Fz = a.pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
-    float Fw = fp.w;
-    if (fpre) fpre[i] = make_float4(Fx, Fy, Fz, 0.0f);  // taped for the adjoint
+        float vfx = a.vfx, vfy = a.vfy, vfz = a.vfz;
+        if (pi.y > P.crest) {
+            fp.x /= P.brk_mass;
+            fp.y /= P.brk_mass;
+            fp.z /= P.brk_mass;
+            fp.w /= P.brk_mass;
+            vfx *= P.brk_visc;
+            vfy *= P.brk_visc;
+            vfz *= P.brk_visc;
+        }
+        vfx *= P.visc;
+        vfy *= P.visc;
+        vfz *= P.visc;
+        const float u = (pi.x - P.org[0]) * P.uvs[0];
+        const float v = (pi.z - P.org[1]) * P.uvs[1];
+        const float height = tex_sample(H0, P.W, P.H, u, v);
+        float tqx = (pi.y * fp.z - pi.z * fp.y) * P.torque;
+        float tqy = (pi.z * fp.x - pi.x * fp.z) * P.torque;
+        float tqz = (pi.x * fp.y - pi.y * fp.x) * P.torque;
+        float drx = 0.f, dry = 0.f, drz = 0.f, wfx = 0.f, wfy = 0.f, wfz = 0.f;
+        if (coupled(P, pi.y, height)) {
+            // WaveVelocity (force_comp.glsl:117-128)
+            const float hx = tex_sample(H0, P.W, P.H, u + P.du, v);
+            const float hy = tex_sample(H0, P.W, P.H, u, v + P.du);
+            const float wvx = (hx - height) / dt, wvy = (hy - height) / dt, wvz = (hx - hy) / P.du;
+            drx = -P.drag * (vi.x - wvx);
+            dry = -P.drag * (vi.y - wvy);
+            drz = -P.drag * (vi.z - wvz);
+            // WaveNormal (force_comp.glsl:131-139) = (H(u+1)-H, H(v+1)-H, -1)
+            const float nx = tex_sample(H0, P.W, P.H, u + 1.0f, v) - height;
+            const float ny = tex_sample(H0, P.W, P.H, u, v + 1.0f) - height;
+            wfx = -height * nx * P.wforce;
+            wfy = -height * ny * P.wforce;
+            wfz = -height * -1.0f * P.wforce;
+        }
+        float rho = rho_w[i];
+        float prs = p_i;
+        float Fx = a.pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
+        float Fy = a.pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
+        float Fz = a.pfz + vfz + rho * P.g[2] + tqz + drz + wfz;
+        float Fw = fp.w;
+        if (fpre) fpre[i] = make_float4(Fx, Fy, Fz, 0.0f);  // taped for the adjoint
 
-    // integrate_comp.glsl:56-92
-    float nvx = vi.x + dt * (Fx / rho), nvy = vi.y + dt * (Fy / rho), nvz = vi.z + dt * (Fz / rho);
-    float npx = pi.x + dt * nvx, npy = pi.y + dt * nvy, npz = pi.z + dt * nvz;
-    const float damp = 1.0f - P.vdamp * dt;
-    nvx *= damp;
-    nvy *= damp;
-    nvz *= damp;
-    if (sqrtf(dist2_fma(nvx, nvy, nvz)) > P.foam_speed) {
-        Fx *= P.foam_f;
-        Fy *= P.foam_f;
-        Fz *= P.foam_f;
-        Fw *= P.foam_f;
-        rho *= P.foam_rho;
-        prs *= P.foam_p;
-        nvx *= P.foam_v;
-        nvy *= P.foam_v;
-        nvz *= P.foam_v;
-    }
-    const float th = tex_sample(H0, P.W, P.H, (npx - P.org[0]) * P.uvs[0],
-                                (npz - P.org[1]) * P.uvs[1]);
-    if (npy < th) npy = th - P.rp;
-    // CheckBoundary (integrate_comp.glsl:135-178)
-    if (npx < P.lo[0]) { npx = P.lo[0]; nvx *= -P.bounce; }
-    else if (npx > P.up[0]) { npx = P.up[0]; nvx *= -P.bounce; }
-    if (npy < P.lo[1]) { npy = P.lo[1]; nvy *= -P.bounce; }
-    else if (npy > P.up[1]) { npy = P.up[1]; nvy *= -P.bounce; }
-    if (npz < P.lo[2]) { npz = P.lo[2]; nvz *= -P.bounce; }
-    else if (npz > P.up[2]) { npz = P.up[2]; nvz *= -P.bounce; }
-    if (P.wake) {
-        if (npy > P.up[1] + P.ceil_extra) { npy = P.up[1] + P.ceil_extra; nvy *= -P.bounce; }
-    }
-    pos_o[i] = make_float4(npx, npy, npz, pi.w);
-    vel_o[i] = make_float4(nvx, nvy, nvz, vi.w);
-    force_o[i] = make_float4(Fx, Fy, Fz, Fw);
-    id_o[i] = id[i];
-    rho_o[i] = rho;
-    prs_o[i] = prs;
+        // integrate_comp.glsl:56-92
+        float nvx = vi.x + dt * (Fx / rho), nvy = vi.y + dt * (Fy / rho), nvz = vi.z + dt * (Fz / rho);
+        float npx = pi.x + dt * nvx, npy = pi.y + dt * nvy, npz = pi.z + dt * nvz;
+        const float damp = 1.0f - P.vdamp * dt;
+        nvx *= damp;
+        nvy *= damp;
+        nvz *= damp;
+        if (sqrtf(dist2_fma(nvx, nvy, nvz)) > P.foam_speed) {
+            Fx *= P.foam_f;
+            Fy *= P.foam_f;
+            Fz *= P.foam_f;
+            Fw *= P.foam_f;
+            rho *= P.foam_rho;
+            prs *= P.foam_p;
+            nvx *= P.foam_v;
+            nvy *= P.foam_v;
+            nvz *= P.foam_v;
+        }
+        const float th = tex_sample(H0, P.W, P.H, (npx - P.org[0]) * P.uvs[0],
+                                    (npz - P.org[1]) * P.uvs[1]);
+        if (npy < th) npy = th - P.rp;
+        // CheckBoundary (integrate_comp.glsl:135-178)
+        if (npx < P.lo[0]) { npx = P.lo[0]; nvx *= -P.bounce; }
+        else if (npx > P.up[0]) { npx = P.up[0]; nvx *= -P.bounce; }
+        if (npy < P.lo[1]) { npy = P.lo[1]; nvy *= -P.bounce; }
+        else if (npy > P.up[1]) { npy = P.up[1]; nvy *= -P.bounce; }
+        if (npz < P.lo[2]) { npz = P.lo[2]; nvz *= -P.bounce; }
+        else if (npz > P.up[2]) { npz = P.up[2]; nvz *= -P.bounce; }
+        if (P.wake) {
+            if (npy > P.up[1] + P.ceil_extra) { npy = P.up[1] + P.ceil_extra; nvy *= -P.bounce; }
+        }
+        pos_o[i] = make_float4(npx, npy, npz, pi.w);
+        vel_o[i] = make_float4(nvx, nvy, nvz, vi.w);
+        force_o[i] = make_float4(Fx, Fy, Fz, Fw);
+        id_o[i] = bid[i];
+        rho_o[i] = rho;
+        prs_o[i] = prs;
+        if (next.key) {
+            const int ix = cell_axis(npx, P.gmin[0], P.cell, P.dims[0]);
+            const int iy = cell_axis(npy, P.gmin[1], P.cell_y, P.dims[1]);
+            const int iz = cell_axis(npz, P.gmin[2], P.cell, P.dims[2]);
+            cell = lin_cell(P.dims, ix, iy, iz);
+            next.key[i] = cell;
+        }
+    }  // valid
+    if (next.key) count_runs(cell, valid, next.count);
 }
 
 // ---------------------------------------------------------------------------
@@ -998,35 +1030,35 @@ hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* rec,
-                             int* count, hipStream_t st) {
+hipError_t launch_hash_count(const float4* pos, int n, const GridDesc& g, int* key, int* count,
+                             hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(pos, n, g, key, rec, count);
+    k_hash_count<<<blocks_for(n, 256), 256, 0, st>>>(pos, n, g, key, count);
     return hipGetLastError();
 }
 
-hipError_t launch_key_count(const int* key, int n, int* rec, int* count, hipStream_t st) {
+hipError_t launch_key_count(const int* key, int n, int* count, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_key_count<<<blocks_for(n, 256), 256, 0, st>>>(key, n, rec, count);
+    k_key_count<<<blocks_for(n, 256), 256, 0, st>>>(key, n, count);
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_runs(const int* key, const int* rec, const int* offset, int n,
-                               int2* runs, hipStream_t st) {
+hipError_t launch_scatter_runs(const int* key, const int* offset, int* fill, int n, int2* runs,
+                               hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_scatter_runs<<<blocks_for(n, 256), 256, 0, st>>>(key, rec, offset, n, runs);
+    k_scatter_runs<<<blocks_for(n, 256), 256, 0, st>>>(key, offset, fill, n, runs);
     return hipGetLastError();
 }
 
-hipError_t launch_stable_reorder(const int* key, const int* rec, const int* offset,
-                                 const int2* runs, int n, const int* item_of, int* list,
-                                 const ParticleArrays* in, ParticleArrays* out, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    k_stable_reorder<<<blocks_for(n, 256), 256, 0, st>>>(
-        key, rec, offset, runs, n, item_of, list, in ? in->pos : nullptr,
-        in ? in->vel : nullptr, in ? in->force : nullptr, in ? in->id : nullptr,
-        out ? out->pos : nullptr, out ? out->vel : nullptr, out ? out->force : nullptr,
-        out ? out->id : nullptr);
+hipError_t launch_stable_reorder(const int* key, const int* offset, const int2* runs, int n,
+                                 const int* item_of, int* list, const float4* pos_in,
+                                 const int* id_in, float4* pos_out, int* id_out, int* fill,
+                                 int nfill, hipStream_t st) {
+    const int m = std::max(n, nfill);
+    if (m <= 0) return hipSuccess;
+    k_stable_reorder<<<blocks_for(m, 256), 256, 0, st>>>(key, offset, runs, n, item_of, list,
+                                                         pos_in, id_in, pos_out, id_out, fill,
+                                                         nfill);
     return hipGetLastError();
 }
 
@@ -1063,31 +1095,37 @@ hipError_t launch_box_pairs(const float4* x, const float4* hw, int n, const floa
     return hipGetLastError();
 }
 
-hipError_t launch_density(const DevParams& P, const ParticleArrays& B, const int* offset, int n,
-                          const float* H0, float* rho_w, float2* pw, float* flux, uint2* nbw,
-                          float4* nrec, float* rho_raw, hipStream_t st) {
+hipError_t launch_density(const DevParams& P, const float4* bpos, const float4* state_vel,
+                          const int* list, const int* offset, int n, const float* H0,
+                          float* rho_w, float2* pw, float* flux, uint2* nbw, float4* nrec,
+                          float* rho_raw, float4* vel_out, float* vpad, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     if (n <= kDensitySplitMaxN)
         k_density_split<<<blocks_for((long long)n * kDensitySplit, 256), 256, 0, st>>>(
-            P.dev, B.pos, B.vel, offset, n, H0, rho_w, pw, flux, nbw, nrec, rho_raw);
+            P.dev, bpos, state_vel, list, offset, n, H0, rho_w, pw, flux, nbw, nrec, rho_raw,
+            vel_out, vpad);
     else
-        k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, B.pos, B.vel, offset, n, H0, rho_w,
-                                                      pw, flux, nbw, nrec, rho_raw);
+        k_density<<<blocks_for(n, 256), 256, 0, st>>>(P.dev, bpos, state_vel, list, offset, n,
+                                                      H0, rho_w, pw, flux, nbw, nrec, rho_raw,
+                                                      vel_out, vpad);
     return hipGetLastError();
 }
 
-hipError_t launch_force_integrate(const DevParams& P, float dt, const ParticleArrays& B,
-                                  const int* offset, int n, const float* rho_w, const float2* pw,
-                                  const uint2* nbw, const float4* nrec, const float* H0,
-                                  ParticleArrays& A, float* rho, float* prs, float4* fpre,
-                                  const float* flux, float* colflux, hipStream_t st) {
+hipError_t launch_force_integrate(const DevParams& P, float dt, const float4* bpos,
+                                  const int* bid, const ParticleArrays& S, const int* list,
+                                  const int* offset, int n, const float* rho_w, const uint2* nbw,
+                                  const float4* nrec, const float* H0, ParticleArrays& N,
+                                  float* rho, float* prs, float4* fpre, const float* flux,
+                                  float* colflux, const float* vpad, const NextKeys& next,
+                                  hipStream_t st) {
     if (n <= 0) return hipSuccess;
     const int nfb = blocks_for(n, 256);
     const long long ncol = colflux ? (long long)P.dims[0] * P.dims[2] : 0;
     const int ncb = blocks_for(ncol * 64, 256);
-    k_force_integrate<<<nfb + ncb, 256, 0, st>>>(P.dev, dt, B.pos, B.vel, B.force, B.id, offset, n,
-                                                 rho_w, pw, nbw, nrec, H0, A.pos, A.vel, A.force,
-                                                 A.id, rho, prs, fpre, nfb, flux, colflux);
+    k_force_integrate<<<nfb + ncb, 256, 0, st>>>(P.dev, dt, bpos, bid, vpad, S.force, list,
+                                                 offset, n, rho_w, nbw, nrec, H0, N.pos, N.vel,
+                                                 N.force, N.id, rho, prs, fpre, nfb, flux,
+                                                 colflux, next);
     return hipGetLastError();
 }
 

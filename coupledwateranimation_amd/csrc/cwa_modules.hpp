@@ -19,6 +19,7 @@
 #include <list>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/cwa.h"
@@ -115,42 +116,60 @@ class ParallelScan {
 };
 
 // Ugrid3D / UgridParticles3D: per-step counting sort of particle slots by
-// cell with a stable (slot-ordered) insertion, optionally moving the
-// particle state into sorted order.
+// cell with a stable (slot-ordered) insertion, moving the positions and ids
+// into sorted order.  The cell keys of a build come from its own hash pass,
+// or from the force pass of the step before, which hashes the positions it
+// writes (FusedNext / KeysFused); any change of the particles or counters
+// between the two (ClearCounters) drops those keys.
 class UniformGrid3D {
    public:
     void Init(const GridDesc& g, int max_particles);
-    // BuildGrid (UniformGridParticles3D.cpp:170-211) over pos[0..n).
-    // list_out / offset_out (optional) replace the grid's own list / offsets.
-    void BuildGrid(const float4* pos, int n, const ParticleArrays* in, ParticleArrays* out,
-                   hipStream_t st, hipEvent_t* ev = nullptr, int* list_out = nullptr,
+    // BuildGrid (UniformGridParticles3D.cpp:170-211) over A.pos[0..n):
+    // sorted positions / ids into bpos / bid.  list_out / offset_out
+    // (optional) replace the grid's own list / offsets.
+    void BuildGrid(const ParticleArrays& A, int n, float4* bpos, int* bid, hipStream_t st,
+                   hipEvent_t* ev = nullptr, int* list_out = nullptr,
                    int* offset_out = nullptr);
+    // Where the force pass of this step hashes the next step's keys.
+    NextKeys FusedNext() const { return NextKeys{key_[kc_ ^ 1].get(), count_[cur_].get()}; }
+    // The force pass has written the next step's keys and counts.
+    void KeysFused() {
+        kc_ ^= 1;
+        keys_ready_ = true;
+    }
+    bool keys_ready() const { return keys_ready_; }
     const GridDesc& desc() const { return g_; }
     int num_cells() const { return g_.dims[0] * g_.dims[1] * g_.dims[2]; }
-    int* key() const { return key_.get(); }
+    // keys of the last build
+    int* key() const { return key_[last_kc_].get(); }
     // counts of the last build (the other half of the double buffer is the
     // zeroed counter of the next build)
     int* count() const { return count_[last_].get(); }
     int* offset() const { return offset_.get(); }
     int* list() const { return list_.get(); }
     // Zero both counters (the counter of the next build must start at zero;
-    // each build's scan clears the other one).
+    // each build's scan clears the other one); drops keys hashed ahead.
     void ClearCounters(hipStream_t st);
-    int parity() const { return cur_; }
-    void get_parity(int& cur, int& last) const {
-        cur = cur_;
-        last = last_;
-    }
-    void set_parity(int cur, int last) {
-        cur_ = cur;
-        last_ = last;
+    struct Parity {
+        int cur, last, kc, last_kc, keys_ready;
+    };
+    int parity() const { return cur_ * 4 + kc_ * 2 + (keys_ready_ ? 1 : 0); }
+    Parity get_parity() const { return Parity{cur_, last_, kc_, last_kc_, keys_ready_ ? 1 : 0}; }
+    void set_parity(const Parity& p) {
+        cur_ = p.cur;
+        last_ = p.last;
+        kc_ = p.kc;
+        last_kc_ = p.last_kc;
+        keys_ready_ = p.keys_ready != 0;
     }
 
    private:
     GridDesc g_{};
-    DeviceBuffer<int> key_, rec_, count_[2], list_, offset_, ws_;
+    DeviceBuffer<int> key_[2], count_[2], fill_, list_, offset_, ws_;
     DeviceBuffer<int2> runs_;
-    int cur_ = 0, last_ = 0;
+    int cur_ = 0, last_ = 0;       // counter double buffer
+    int kc_ = 0, last_kc_ = 0;     // key double buffer
+    bool keys_ready_ = false;      // key_[kc_] / count_[cur_] hold the next build's hash
 };
 
 // StencilImage2DTripleBuffered: three fp32 fields rotated by PingPong.
@@ -195,30 +214,43 @@ class HeightField2DTripleBuffered {
     DeviceBuffer<float> img_[3];
 };
 
-// The particle state (A), its sorted reorder target (B), per-step work
-// arrays and the visible density / pressure.
+// The particle state in two copies (A: current, N: next; each step reads A
+// and writes N in its sorted order, then they swap), the sorted positions
+// and ids of the step (B), per-step work arrays and the visible density /
+// pressure.
 class SphParticles3D {
    public:
     void Allocate(int n);
     void Upload(const float* pos, const float* vel, const float* force, hipStream_t st);
     int n() const { return n_; }
-    ParticleArrays A, B;
+    ParticleArrays A, N;
+    float4* bpos() const { return bpos_.get(); }
+    int* bid() const { return bid_.get(); }
+    // N becomes the current state
+    void Flip() {
+        std::swap(A, N);
+        parity_ ^= 1;
+    }
+    int parity() const { return parity_; }
+    void set_parity(int p) {
+        if (p != parity_) Flip();
+    }
     float* rho() const { return rho_.get(); }
     float* prs() const { return prs_.get(); }
     float* rho_w() const { return rho_w_.get(); }
-    float2* pw() const { return pw_.get(); }
     float* flux() const { return flux_.get(); }
+    float* vpad() const { return vpad_.get(); }
     uint2* nbw() const { return nbw_.get(); }
     float4* nrec() const { return nrec_.get(); }
 
    private:
     int n_ = 0;
+    int parity_ = 0;
     DeviceBuffer<uint2> nbw_;  // neighbour words and counts of the last step
     DeviceBuffer<float4> nrec_;   // 2 x n neighbour records of the last step
-    DeviceBuffer<float4> pos_[2], vel_[2], force_[2];
-    DeviceBuffer<int> id_[2];
-    DeviceBuffer<float> rho_, prs_, rho_w_, flux_;
-    DeviceBuffer<float2> pw_;
+    DeviceBuffer<float4> pos_[2], vel_[2], force_[2], bpos_;
+    DeviceBuffer<int> id_[2], bid_;
+    DeviceBuffer<float> rho_, prs_, rho_w_, flux_, vpad_;
 };
 
 // Optional per-step output locations used by the adjoint tape: when set,
@@ -234,7 +266,7 @@ struct TapeOut {
     uint2* nbw = nullptr;      // neighbour words
     float4* nrec = nullptr;    // neighbour records
     float* rho = nullptr;      // density (after the clamp)
-    float2* pw = nullptr;      // (pressure, 1/rho)
+    float2* pw = nullptr;      // (pressure, 1/rho), written only for the tape
     float* colflux = nullptr;  // column flux
     // heightfields: when h_out is set the step reads h_in0 (newest) and
     // h_in1 (previous) instead of the scene's triple buffer and writes the
@@ -292,25 +324,29 @@ class CoupledScene : public Module {
     // captured code paths of this scene (tapes, fits) record it and refuse
     // to run once it has moved.
     unsigned long long layout_generation() const { return layout_gen_; }
-    // Host-side phase of the buffer rotations (wave triple buffer x grid
-    // counter double buffer), period 6.
-    int phase() const { return wave.phase() * 2 + grid.parity(); }
+    // Host-side phase of the buffer rotations (wave triple buffer, grid
+    // counter and key double buffers, particle state double buffer): in a
+    // run of steps with fused keys the period is 6.
+    int phase() const { return (wave.phase() * 8 + grid.parity()) * 2 + particles.parity(); }
     // The host-side rotation state itself: a captured sequence replays the
     // buffers of the rotation it was captured at, so its owner puts the
     // scene back into that rotation around replays.
     struct Rotation {
         int wave[3];
-        int grid_cur, grid_last;
+        UniformGrid3D::Parity grid;
+        int particles;
     };
     Rotation rotation() const {
         Rotation r{};
         wave.get_rotation(r.wave);
-        grid.get_parity(r.grid_cur, r.grid_last);
+        r.grid = grid.get_parity();
+        r.particles = particles.parity();
         return r;
     }
     void set_rotation(const Rotation& r) {
         wave.set_rotation(r.wave);
-        grid.set_parity(r.grid_cur, r.grid_last);
+        grid.set_parity(r.grid);
+        particles.set_parity(r.particles);
     }
 
     SphParticles3D particles;

@@ -61,25 +61,29 @@ void UniformGrid3D::Init(const GridDesc& g, int max_particles) {
     g_ = g;
     const size_t nc = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
     const size_t np = (size_t)std::max(max_particles, 1);
-    key_.resize(np);
-    rec_.resize(np);
+    for (int k = 0; k < 2; ++k) {
+        key_[k].resize(np);
+        count_[k].resize(nc);
+    }
+    fill_.resize(nc);
     runs_.resize(np);
     list_.resize(np);
-    count_[0].resize(nc);
-    count_[1].resize(nc);
     offset_.resize(nc + 1);
     ws_.resize(scan_workspace_ints((int)nc));
     cur_ = last_ = 0;
+    kc_ = last_kc_ = 0;
+    keys_ready_ = false;
 }
 
 void UniformGrid3D::ClearCounters(hipStream_t st) {
     for (auto& c : count_) hip_check(launch_fill_i32(c.get(), num_cells(), 0, st), "clear counts");
+    hip_check(launch_fill_i32(fill_.get(), num_cells(), 0, st), "clear fill");
     hip_check(launch_fill_i32(offset_.get(), num_cells() + 1, 0, st), "clear offsets");
+    keys_ready_ = false;
 }
 
-void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in,
-                              ParticleArrays* out, hipStream_t st, hipEvent_t* ev,
-                              int* list_out, int* offset_out) {
+void UniformGrid3D::BuildGrid(const ParticleArrays& A, int n, float4* bpos, int* bid,
+                              hipStream_t st, hipEvent_t* ev, int* list_out, int* offset_out) {
     const int nc = num_cells();
     int* lst = list_out ? list_out : list_.get();
     int* off = offset_out ? offset_out : offset_.get();
@@ -87,20 +91,24 @@ void UniformGrid3D::BuildGrid(const float4* pos, int n, const ParticleArrays* in
         if (ev) hip_check(hipEventRecord(ev[2 * slot + which], st), "event");
     };
     int* cnt = count_[cur_].get();  // zero: cleared by the previous build's scan
+    int* key = key_[kc_].get();
     mark(K_HASH, 0);
-    hip_check(launch_hash_count(pos, n, g_, key_.get(), rec_.get(), cnt, st), "hash");
+    if (!keys_ready_)  // else the last force pass hashed these positions
+        hip_check(launch_hash_count(A.pos, n, g_, key, cnt, st), "hash");
     mark(K_HASH, 1);
+    keys_ready_ = false;
     mark(K_SCAN, 0);
     hip_check(launch_scan(cnt, off, nc, ws_.get(), st, count_[cur_ ^ 1].get()), "scan");
     mark(K_SCAN, 1);
     last_ = cur_;
     cur_ ^= 1;
+    last_kc_ = kc_;
     mark(K_SCATTER, 0);
-    hip_check(launch_scatter_runs(key_.get(), rec_.get(), off, n, runs_.get(), st), "scatter");
+    hip_check(launch_scatter_runs(key, off, fill_.get(), n, runs_.get(), st), "scatter");
     mark(K_SCATTER, 1);
     mark(K_REORDER, 0);
-    hip_check(launch_stable_reorder(key_.get(), rec_.get(), off, runs_.get(), n, nullptr, lst, in,
-                                    out, st),
+    hip_check(launch_stable_reorder(key, off, runs_.get(), n, nullptr, lst, A.pos, A.id, bpos, bid,
+                                    fill_.get(), nc, st),
               "reorder");
     mark(K_REORDER, 1);
 }
@@ -186,15 +194,18 @@ void SphParticles3D::Allocate(int n) {
         force_[k].resize(m);
         id_[k].resize(m);
     }
+    bpos_.resize(m);
+    bid_.resize(m);
     rho_.resize(m);
     prs_.resize(m);
     rho_w_.resize(m);
     flux_.resize(m);
-    pw_.resize(m);
+    vpad_.resize(m);
     nbw_.resize(nb_buffer_uint2(m));
     nrec_.resize(2 * m);
     A = ParticleArrays{pos_[0].get(), vel_[0].get(), force_[0].get(), id_[0].get()};
-    B = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
+    N = ParticleArrays{pos_[1].get(), vel_[1].get(), force_[1].get(), id_[1].get()};
+    parity_ = 0;
 }
 
 void SphParticles3D::Upload(const float* pos, const float* vel, const float* force,
@@ -448,32 +459,35 @@ void CoupledScene::Step(float dt, hipEvent_t (*ev)[2], const TapeOut* tape) {
     const bool couple = n > 0 && params_.couple_band > 0.0f;
     // where this step's arrays live (the scene's own, or the tape's)
     auto pick = [&](auto* t, auto* own) { return (tape && t) ? t : own; };
-    ParticleArrays B = particles.B;
-    B.pos = pick(tape ? tape->pos : nullptr, particles.B.pos);
-    B.vel = pick(tape ? tape->vel : nullptr, particles.B.vel);
+    float4* bpos = pick(tape ? tape->pos : nullptr, particles.bpos());
     int* off = pick(tape ? tape->offset : nullptr, grid.offset());
     int* lst = pick(tape ? tape->list : nullptr, grid.list());
     uint2* nbm = pick(tape ? tape->nbw : nullptr, particles.nbw());
     float4* nrec = pick(tape ? tape->nrec : nullptr, particles.nrec());
     float* rho_w = pick(tape ? tape->rho : nullptr, particles.rho_w());
-    float2* pw = pick(tape ? tape->pw : nullptr, particles.pw());
     float* cflux = pick(tape ? tape->colflux : nullptr, colflux.get());
     if (n > 0) {
-        grid.BuildGrid(particles.A.pos, n, &particles.A, &B, st_, ev ? &ev[0][0] : nullptr, lst,
+        grid.BuildGrid(particles.A, n, bpos, particles.bid(), st_, ev ? &ev[0][0] : nullptr, lst,
                        off);
         mark(K_DENSITY, 0);
-        hip_check(launch_density(dp_, B, off, n, H0, rho_w, pw, particles.flux(), nbm, nrec,
-                                 tape ? tape->rho_raw : nullptr, st_),
+        hip_check(launch_density(dp_, bpos, particles.A.vel, lst, off, n, H0, rho_w,
+                                 tape ? tape->pw : nullptr, particles.flux(), nbm, nrec,
+                                 tape ? tape->rho_raw : nullptr, tape ? tape->vel : nullptr,
+                                 particles.vpad(), st_),
                   "density");
         mark(K_DENSITY, 1);
-        // the same launch sums the particle flux per column when coupled
+        // the same launch sums the particle flux per column when coupled, and
+        // hashes the new positions into the next step's grid
         mark(K_FORCE, 0);
-        hip_check(launch_force_integrate(dp_, dt, B, off, n, rho_w, pw, nbm, nrec, H0,
-                                         particles.A, particles.rho(), particles.prs(),
-                                         tape ? tape->fpre : nullptr, particles.flux(),
-                                         couple ? cflux : nullptr, st_),
+        hip_check(launch_force_integrate(dp_, dt, bpos, particles.bid(), particles.A, lst, off, n,
+                                         rho_w, nbm, nrec, H0, particles.N, particles.rho(),
+                                         particles.prs(), tape ? tape->fpre : nullptr,
+                                         particles.flux(), couple ? cflux : nullptr,
+                                         particles.vpad(), grid.FusedNext(), st_),
                   "force/integrate");
         mark(K_FORCE, 1);
+        particles.Flip();
+        grid.KeysFused();
     }
     mark(K_WAVE, 0);
     if (taped_heights)
@@ -516,7 +530,9 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
         for (int k = 0; k < nsteps; ++k) Step(dt);
         return;
     }
-    const int cycle = 6;  // wave triple buffer (3) x grid counter double buffer (2)
+    // wave triple buffer (3) x the double buffers (2) of the grid counters,
+    // the grid keys and the particle state
+    const int cycle = 6;
     int done = 0;
     // A captured graph replays the buffers of the phase it was captured at:
     // step eagerly until the rotation phase matches it again.
@@ -526,7 +542,16 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
             ++done;
         }
     }
-    if (nsteps - done >= cycle && (!gexec_ || graph_dt_ != dt)) {
+    // A capture starts with the keys hashed ahead by the last force pass, so
+    // that no captured step has a hash pass of its own and every replay can
+    // follow the one before.
+    if (nsteps - done > cycle && (!gexec_ || graph_dt_ != dt) && particles.n() > 0 &&
+        !grid.keys_ready()) {
+        Step(dt);
+        ++done;
+    }
+    if (nsteps - done >= cycle && (!gexec_ || graph_dt_ != dt) &&
+        (particles.n() == 0 || grid.keys_ready())) {
         invalidate_graph();
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
@@ -575,7 +600,7 @@ void CoupledScene::NeighbourCounts(int32_t* out) {
         throw Error(CWA_ESTATE,
                     "neighbour counts exist after a step, until the next reinit or set_params");
     DeviceBuffer<int> d(n);
-    hip_check(launch_neighbour_counts(dp_, particles.B.pos, grid.offset(), particles.nbw(), n,
+    hip_check(launch_neighbour_counts(dp_, particles.bpos(), grid.offset(), particles.nbw(), n,
                                       d.get(), st_),
               "neighbour counts");
     hip_check(hipStreamSynchronize(st_), "sync");
@@ -973,16 +998,17 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
         g.cell = cell_size;
         g.cell_y = cell_size;
         const int nc = dims[0] * dims[1] * dims[2];
-        DeviceBuffer<int> rec((size_t)std::max(n, 1)), ws(scan_workspace_ints(nc));
+        DeviceBuffer<int> fill(nc), ws(scan_workspace_ints(nc));
         DeviceBuffer<int2> runs((size_t)std::max(n, 1));
         hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
-        hip_check(launch_hash_count(reinterpret_cast<const float4*>(d_pos4), n, g, d_key,
-                                    rec.get(), d_count, st),
+        hip_check(launch_fill_i32(fill.get(), nc, 0, st), "clear fill");
+        hip_check(launch_hash_count(reinterpret_cast<const float4*>(d_pos4), n, g, d_key, d_count,
+                                    st),
                   "hash");
         hip_check(launch_scan(d_count, d_offset, nc, ws.get(), st), "scan");
-        hip_check(launch_scatter_runs(d_key, rec.get(), d_offset, n, runs.get(), st), "scatter");
-        hip_check(launch_stable_reorder(d_key, rec.get(), d_offset, runs.get(), n, nullptr, d_list,
-                                        nullptr, nullptr, st),
+        hip_check(launch_scatter_runs(d_key, d_offset, fill.get(), n, runs.get(), st), "scatter");
+        hip_check(launch_stable_reorder(d_key, d_offset, runs.get(), n, nullptr, d_list, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr, 0, st),
                   "stable insert");
         hip_check(hipStreamSynchronize(st), "grid sync");
     });
@@ -1029,18 +1055,20 @@ int cwa_grid_build_boxes(const float* d_centre4, const float* d_halfw4, int n,
         const int npairs = (int)npairs64;
         hip_check(launch_scan(ncov.get(), first.get(), n, ws.get(), st), "cover scan");
         const size_t np = (size_t)std::max(npairs, 1);
-        DeviceBuffer<int> pkey(np), pbox(np), rec(np);
+        DeviceBuffer<int> pkey(np), pbox(np), fill(nc);
         DeviceBuffer<int2> runs(np);
         hip_check(launch_box_pairs(x, hw, n, grid_min, cell_size, dims, first.get(), pkey.get(),
                                    pbox.get(), st),
                   "pairs");
         hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
-        hip_check(launch_key_count(pkey.get(), npairs, rec.get(), d_count, st), "count");
+        hip_check(launch_fill_i32(fill.get(), nc, 0, st), "clear fill");
+        hip_check(launch_key_count(pkey.get(), npairs, d_count, st), "count");
         hip_check(launch_scan(d_count, d_offset, nc, ws.get(), st), "scan");
-        hip_check(launch_scatter_runs(pkey.get(), rec.get(), d_offset, npairs, runs.get(), st),
+        hip_check(launch_scatter_runs(pkey.get(), d_offset, fill.get(), npairs, runs.get(), st),
                   "scatter");
-        hip_check(launch_stable_reorder(pkey.get(), rec.get(), d_offset, runs.get(), npairs,
-                                        pbox.get(), d_list, nullptr, nullptr, st),
+        hip_check(launch_stable_reorder(pkey.get(), d_offset, runs.get(), npairs, pbox.get(),
+                                        d_list, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                        st),
                   "stable insert");
         hip_check(hipStreamSynchronize(st), "box grid sync");
     });
