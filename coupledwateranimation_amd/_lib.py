@@ -196,6 +196,9 @@ SIGNATURES = {
     "cwa_grid_build": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_float,
                                  C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]),
+    "cwa_grid_build_static": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_float,
+                                        C.POINTER(C.c_int32), C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "cwa_grid_build_boxes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_int64,

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -645,6 +646,46 @@ void CoupledScene::Readback(cwa_host_state* h) {
 }  // namespace cwa
 
 // ===========================================================================
+// Scratch of the allocation-free grid build (cwa_grid_build_static): run
+// records for up to kStaticMaxItems items, fill counters for up to
+// kStaticMaxCells cells and the cell scan's workspace, in one file-scope
+// device array that every GPU running the library holds until the process
+// ends (32 MB).  Host code never passes the array's name to a kernel: its
+// device address, looked up once per device, is what the kernels get.
+// ===========================================================================
+namespace cwa {
+constexpr int kStaticMaxItems = 1 << 21;
+constexpr int kStaticMaxCells = 1 << 22;
+constexpr size_t kStaticRunsBytes = (size_t)kStaticMaxItems * sizeof(int2);
+constexpr size_t kStaticFillBytes = (size_t)kStaticMaxCells * sizeof(int);
+constexpr size_t kStaticWsInts = 8192;  // scan_workspace_ints(kStaticMaxCells) is 4099
+}  // namespace cwa
+
+static __device__ __align__(256) unsigned char scratch[cwa::kStaticRunsBytes +
+                                                       cwa::kStaticFillBytes +
+                                                       cwa::kStaticWsInts * sizeof(int)];
+
+namespace cwa {
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, under a lock).
+static unsigned char* static_scratch() {
+    static std::mutex mu;
+    static std::vector<unsigned char*> per_device;
+    int dev = -1;
+    hip_check(hipGetDevice(&dev), "current device");
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+        if (!p) throw Error(CWA_EHIP, "scratch address is null");
+        per_device[dev] = static_cast<unsigned char*>(p);
+    }
+    return per_device[dev];
+}
+}  // namespace cwa
+
+// ===========================================================================
 // C ABI
 // ===========================================================================
 using namespace cwa;
@@ -1011,6 +1052,47 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
                                         nullptr, nullptr, nullptr, nullptr, 0, st),
                   "stable insert");
         hip_check(hipStreamSynchronize(st), "grid sync");
+    });
+}
+
+int cwa_grid_build_static(const float* d_pos4, int n, const float grid_min[3], float cell_size,
+                          const int32_t dims[3], int32_t* d_key, int32_t* d_count,
+                          int32_t* d_offset, int32_t* d_list, void* stream) {
+    return guarded([&] {
+        CWA_REQUIRE(n >= 0, "n < 0");
+        CWA_REQUIRE(grid_min && dims && d_count && d_offset, "null argument");
+        CWA_REQUIRE(n == 0 || (d_pos4 && d_key && d_list), "null argument");
+        CWA_REQUIRE(cell_size > 0.0f, "cell_size must be > 0");
+        CWA_REQUIRE(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, "dims must be >= 1");
+        CWA_REQUIRE(n <= kStaticMaxItems, "n exceeds the static scratch (2^21 particles)");
+        CWA_REQUIRE((double)dims[0] * dims[1] * dims[2] <= (double)kStaticMaxCells,
+                    "the grid exceeds the static scratch (2^22 cells)");
+        const int nc = dims[0] * dims[1] * dims[2];
+        if (scan_workspace_ints(nc) > kStaticWsInts)
+            throw Error(CWA_EINVAL, "scan workspace exceeds the static scratch");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        GridDesc g{};
+        for (int a = 0; a < 3; ++a) {
+            g.gmin[a] = grid_min[a];
+            g.dims[a] = dims[a];
+        }
+        g.cell = cell_size;
+        g.cell_y = cell_size;
+        unsigned char* base = static_scratch();
+        int2* runs = reinterpret_cast<int2*>(base);
+        int* fill = reinterpret_cast<int*>(base + kStaticRunsBytes);
+        int* ws = reinterpret_cast<int*>(base + kStaticRunsBytes + kStaticFillBytes);
+        // the array keeps the last call's contents: clear what must start at zero
+        hip_check(launch_fill_i32(d_count, nc, 0, st), "clear");
+        hip_check(launch_fill_i32(fill, nc, 0, st), "clear fill");
+        hip_check(launch_hash_count(reinterpret_cast<const float4*>(d_pos4), n, g, d_key, d_count,
+                                    st),
+                  "hash");
+        hip_check(launch_scan(d_count, d_offset, nc, ws, st), "scan");
+        hip_check(launch_scatter_runs(d_key, d_offset, fill, n, runs, st), "scatter");
+        hip_check(launch_stable_reorder(d_key, d_offset, runs, n, nullptr, d_list, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr, 0, st),
+                  "stable insert");
     });
 }
 

@@ -264,6 +264,21 @@ int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float ce
                    const int32_t dims[3], int32_t* d_key, int32_t* d_count,
                    int32_t* d_offset, int32_t* d_list, void* stream);
 
+/* cwa_grid_build without allocating: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  Same arguments and results as cwa_grid_build, but its
+ * scratch (run records, fill counters, the scan's workspace) is one
+ * file-scope device array of the library (32 MB per GPU, held until the
+ * process ends), so n <= 2^21 and dims[0]*dims[1]*dims[2] <= 2^22, else
+ * CWA_EINVAL.  Asynchronous on `stream` (no synchronisation; results are
+ * ready when the stream reaches them).  The first call on a device looks
+ * up the array's address; make it outside graph capture.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race. */
+int cwa_grid_build_static(const float* d_pos4, int n, const float grid_min[3], float cell_size,
+                          const int32_t dims[3], int32_t* d_key, int32_t* d_count,
+                          int32_t* d_offset, int32_t* d_list, void* stream);
+
 /* UniformGridPbdGpu3D::CollisionQuery's grid build (UniformGridGpu3D.cpp:
  * 121-162; uniform_grid_pbd_3D_cs.glsl ComputeCounts / ComputeCellIndex /
  * Index / InsertBox :145-206): box i spans the cells
