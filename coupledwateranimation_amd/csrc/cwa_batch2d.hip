@@ -13,7 +13,7 @@
 //                                Wave1D_cs.glsl IterateWave :120-130, FreeBC :139-157
 //
 // Packing.  Scene s owns the slot range [pbeg_s, pbeg_s + n_s) and the
-// global cells [s*C, (s+1)*C) (C = nx*ny cells of size H, y fastest), so
+// global cells [s*C, (s+1)*C) (C = nx*ny cells of size H/2, y fastest), so
 // one counting sort over all cells keeps every scene inside its own slot
 // range.  A step is 7 launches for any number of scenes:
 //   k2_hash      cell keys, wave-aggregated count atomics (run heads)
@@ -84,7 +84,11 @@ struct Dev2 {
     float rho0, k, visc_pre, psi, gx, gy;
     float lox, loy, hix, hiy, walls[4];
     float thr, wref, wpress, wdy, wmask, atten, beta, band, kappa;
-    int nx, ny, ncell, nsph;
+    float hc;            // grid cell edge H/2
+    float eps;           // margin of the trimmed candidate rows
+    int nx, ny, ncell;   // grid cells (H/2) per axis, per scene
+    int nxc;             // flux columns (H wide): cell columns 2c, 2c + 1
+    int nsph;
 };
 
 // ---------------------------------------------------------------------------
@@ -173,6 +177,29 @@ __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
     return min(max((int)f, 0), dim - 1);
 }
 
+// Candidate slot range [b, e) of cell column cx for a query at (x, y) in
+// cell column ix: the rows of the column the disc of radius H reaches,
+// widened by a margin far above the rounding of the float bounds, or an
+// empty range when the column lies H or more away -- the oracle's
+// column_rows, operation for operation (so both see the same candidates,
+// in the same order, also for a neighbour moved out of its cell by the
+// density pass's un-penetration).
+__device__ __forceinline__ void column_rows(const Dev2& P, const int* __restrict__ off, int cb,
+                                            float x, float y, int ix, int cx, int& b, int& e) {
+    float dx = 0.0f;
+    if (cx < ix) dx = x - (P.lox + (float)(cx + 1) * P.hc);
+    else if (cx > ix) dx = (P.lox + (float)cx * P.hc) - x;
+    dx = fmaxf(dx - P.eps, 0.0f);
+    b = e = 0;
+    const float d2 = dx * dx;
+    if (!(d2 < P.T)) return;
+    const float ry = sqrt_rn(P.T - d2) + P.eps;
+    const int ylo = cell2(y - ry, P.loy, P.hc, P.ny);
+    const int yhi = cell2(y + ry, P.loy, P.hc, P.ny);
+    b = off[cb + cx * P.ny + ylo];
+    e = off[cb + cx * P.ny + yhi + 1];
+}
+
 // ---------------------------------------------------------------------------
 // grid: hash / count
 // ---------------------------------------------------------------------------
@@ -190,8 +217,8 @@ __global__ void __launch_bounds__(256)
         int s = blk_scene[blockIdx.x];
         while (i >= sc[s].pbeg + sc[s].n) ++s;
         const float4 p = xv[i];
-        const int ix = cell2(p.x, P.lox, P.H, P.nx);
-        const int iy = cell2(p.y, P.loy, P.H, P.ny);
+        const int ix = cell2(p.x, P.lox, P.hc, P.nx);
+        const int iy = cell2(p.y, P.loy, P.hc, P.ny);
         c = s * P.ncell + ix * P.ny + iy;
         key[i] = c;
     }
@@ -357,13 +384,12 @@ __global__ void __launch_bounds__(BS) k2_density(Dev2 P, float dt, DensityArgs A
         float2 rp = A.rpB[i];
         float rho = rp.x, p = rp.y;
         if (!(y > P.hiy)) {
-            const int ix = cell2(x, P.lox, P.H, P.nx);
-            const int iy = cell2(y, P.loy, P.H, P.ny);
-            const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.ny - 1);
+            const int ix = cell2(x, P.lox, P.hc, P.nx);
             const int cb = t.scene * P.ncell;
             float acc = 0.0f;
-            for (int cx = max(ix - 1, 0); cx <= min(ix + 1, P.nx - 1); ++cx) {
-                const int b = A.off[cb + cx * P.ny + ylo], e = A.off[cb + cx * P.ny + yhi + 1];
+            for (int cx = max(ix - 2, 0); cx <= min(ix + 2, P.nx - 1); ++cx) {
+                int b, e;
+                column_rows(P, A.off, cb, x, y, ix, cx, b, e);
                 auto cand = [&](const float4 pj) {
                     const float dx = x - pj.x, dy = y - pj.y;
                     const float r2 = fmaf(dy, dy, dx * dx);
@@ -428,12 +454,11 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
         const Scene2D S = A.sc[t.scene];
         const float ai = A.aj[i].x;
         float apx = 0.0f, apy = 0.0f, avx = 0.0f, avy = 0.0f;
-        const int ix = cell2(x, P.lox, P.H, P.nx);
-        const int iy = cell2(y, P.loy, P.H, P.ny);
-        const int ylo = max(iy - 1, 0), yhi = min(iy + 1, P.ny - 1);
+        const int ix = cell2(x, P.lox, P.hc, P.nx);
         const int cb = t.scene * P.ncell;
-        for (int cx = max(ix - 1, 0); cx <= min(ix + 1, P.nx - 1); ++cx) {
-            const int b = A.off[cb + cx * P.ny + ylo], e = A.off[cb + cx * P.ny + yhi + 1];
+        for (int cx = max(ix - 2, 0); cx <= min(ix + 2, P.nx - 1); ++cx) {
+            int b, e;
+            column_rows(P, A.off, cb, x, y, ix, cx, b, e);
             auto pair = [&](int j, const float4 qj) {
                 const float dx = x - qj.x, dy = y - qj.y;
                 const float r2 = fmaf(dy, dy, dx * dx);
@@ -519,8 +544,8 @@ __global__ void __launch_bounds__(1024)
         // column sums in the fixed order: lane-strided partials, xor butterfly
         const int lane = tid & 63, wv = tid >> 6, nw = bd >> 6;
         const int cb = blockIdx.x * P.ncell;
-        for (int cx = wv; cx < P.nx; cx += nw) {
-            const int b = off[cb + cx * P.ny], e = off[cb + (cx + 1) * P.ny];
+        for (int cx = wv; cx < P.nxc; cx += nw) {  // flux column = cell columns 2cx, 2cx + 1
+            const int b = off[cb + 2 * cx * P.ny], e = off[cb + min(2 * cx + 2, P.nx) * P.ny];
             float s = 0.0f;
             for (int k = b + lane; k < e; k += 64) s += flux[k];
 #pragma unroll
@@ -589,9 +614,9 @@ __global__ void __launch_bounds__(1024)
             const float gx = ((float)k + 0.5f) * cw * P.invH - 0.5f;
             const float f0 = floorf(gx);
             const float al = gx - f0;
-            const float f = clampf_nan(f0, -2.0f, (float)(P.nx + 1));
-            const int c0 = min(max((int)f, 0), P.nx - 1);
-            const int c1 = min(max((int)f + 1, 0), P.nx - 1);
+            const float f = clampf_nan(f0, -2.0f, (float)(P.nxc + 1));
+            const int c0 = min(max((int)f, 0), P.nxc - 1);
+            const int c1 = min(max((int)f + 1, 0), P.nxc - 1);
             const float Sx = (1.0f - al) * col[c0] + al * col[c1];
             A[k] = A[k] + kd * Sx * P.invH;
         }
@@ -676,9 +701,16 @@ static Dev2 make_dev2(const cwa2d_params& p) {
     d.beta = p.beta;
     d.band = p.couple_band;
     d.kappa = p.couple_strength;
-    d.nx = grid_dim(p.lo[0], p.hi[0], p.h);
-    d.ny = grid_dim(p.lo[1], p.hi[1], p.h);
+    d.hc = 0.5f * p.h;
+    d.nx = grid_dim(p.lo[0], p.hi[0], d.hc);
+    d.ny = grid_dim(p.lo[1], p.hi[1], d.hc);
     d.ncell = d.nx * d.ny;
+    d.nxc = grid_dim(p.lo[0], p.hi[0], p.h);
+    {   // margin of the trimmed rows: far above the rounding of the bounds
+        const float ext = std::max({std::fabs(p.lo[0]), std::fabs(p.hi[0]), std::fabs(p.lo[1]),
+                                    std::fabs(p.hi[1]), p.h});
+        d.eps = 4e-6f * ext;
+    }
     d.nsph = std::max(1, (int)p.sph_substeps);
     return d;
 }
@@ -693,7 +725,8 @@ static void validate(const cwa2d_params& p) {
     if (!(p.wave_maskv > 0.0f) || !(p.wave_dy > 0.0f))
         throw Error(CWA_EINVAL, "wave_maskv and wave_dy must be positive");
     if (p.sph_substeps < 1 || p.sph_substeps > 64) throw Error(CWA_EINVAL, "sph_substeps in 1..64");
-    const double cells = (double)grid_dim(p.lo[0], p.hi[0], p.h) * grid_dim(p.lo[1], p.hi[1], p.h);
+    const double cells = (double)grid_dim(p.lo[0], p.hi[0], 0.5f * p.h) *
+                         grid_dim(p.lo[1], p.hi[1], 0.5f * p.h);
     if (cells > (double)(1 << 24)) throw Error(CWA_EINVAL, "grid too fine for the domain (cells per scene > 2^24)");
     if (grid_dim(p.lo[0], p.hi[0], p.h) > 4096) throw Error(CWA_EINVAL, "more than 4096 grid columns");
 }
@@ -755,7 +788,7 @@ class Batch2D {
         const long long cells = (long long)nscenes * dev_.ncell;
         if (cells + 1 > (1ll << 30)) throw Error(CWA_EINVAL, "too many cells in the batch");
         ncells_ = (int)cells;
-        lds_bytes_ = (size_t)(4 * lmax_ + dev_.nx) * sizeof(float);
+        lds_bytes_ = (size_t)(4 * lmax_ + dev_.nxc) * sizeof(float);
         if (lds_bytes_ > 160 * 1024) throw Error(CWA_EINVAL, "wave column and grid exceed LDS");
 
         hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "stream");
@@ -864,7 +897,7 @@ class Batch2D {
         const Dev2 d = make_dev2(p);
         if (d.nx != dev_.nx || d.ny != dev_.ny)
             throw Error(CWA_EINVAL, "parameters change the grid dimensions of the batch");
-        if ((size_t)(4 * lmax_ + d.nx) * sizeof(float) > lds_bytes_)
+        if ((size_t)(4 * lmax_ + d.nxc) * sizeof(float) > lds_bytes_)
             throw Error(CWA_EINVAL, "grid exceeds the wave pass's LDS");
         params_ = p;
         dev_ = d;

@@ -163,9 +163,10 @@ int cwao2_colsum(void* b, int s, double* out) {
     return static_cast<BatchBase*>(b)->colsum(s, out);
 }
 
+// Grid cells (H/2) per axis.
 void cwao2_dims(const cwa2d_params* p, int32_t* nx, int32_t* ny) {
-    *nx = cwao2::grid_dim(p->lo[0], p->hi[0], p->h);
-    *ny = cwao2::grid_dim(p->lo[1], p->hi[1], p->h);
+    *nx = cwao2::grid_dim(p->lo[0], p->hi[0], 0.5f * p->h);
+    *ny = cwao2::grid_dim(p->lo[1], p->hi[1], 0.5f * p->h);
 }
 
 // Pieces, for the closed-form checks (double).

@@ -36,9 +36,11 @@
 //     fixed order (64 lane-strided partial sums, then a xor butterfly) and
 //     added to the newest level after the substeps as kappa dt S(x) / H.
 //   * Pair terms skip coincident particles (r = 0), where rij/r is 0/0.
-//   * The query cells are the 3x3 cells (cell = H) around the particle's
-//     cell; the reference's (pos -+ H) cell box on its 0.3-wide cells selects
-//     the same neighbours within H, in another order.
+//   * The grid's cells are H/2 wide: the query visits the 5 cell columns
+//     ix-2..ix+2 around the particle's column (column order, slots ascending
+//     within a column, rows iy-2..iy+2), the same neighbours within H as the
+//     reference's (pos -+ H) cell box on its 0.3-wide cells, in another
+//     order.  The particle -> wave flux is summed per column pair (H wide).
 #pragma once
 
 #include <algorithm>
@@ -65,7 +67,14 @@ struct K2 {
     R rho0, k, visc_pre, psi, gx, gy;
     R lox, loy, hix, hiy, walls[4];
     R thr, wref, wpress, wdy, wmask, atten, beta, band, kappa;
-    int nx, ny, nsph;
+    R hc;          // grid cell edge H/2
+    int nx, ny;    // grid cells (H/2) per axis
+    int nxc;       // flux columns (H wide): cell columns 2c and 2c + 1
+    int nsph;
+    // candidate-row trimming, in float whatever R is (the same float
+    // arithmetic as the kernels): the domain and the cell edge as floats,
+    // T = the smallest float with sqrt(T) >= H, the bounds' margin
+    float flox, floy, fhc, fT, feps;
     explicit K2(const cwa2d_params& p) {
         H = R(p.h);
         HSQ = H * H;
@@ -95,9 +104,20 @@ struct K2 {
         beta = R(p.beta);
         band = R(p.couple_band);
         kappa = R(p.couple_strength);
-        nx = grid_dim(p.lo[0], p.hi[0], p.h);
-        ny = grid_dim(p.lo[1], p.hi[1], p.h);
+        hc = R(0.5f * p.h);
+        nx = grid_dim(p.lo[0], p.hi[0], 0.5f * p.h);
+        ny = grid_dim(p.lo[1], p.hi[1], 0.5f * p.h);
+        nxc = grid_dim(p.lo[0], p.hi[0], p.h);
         nsph = std::max(1, (int)p.sph_substeps);
+        flox = p.lo[0];
+        floy = p.lo[1];
+        fhc = 0.5f * p.h;
+        fT = p.h * p.h;
+        while (fT > 0.0f && std::sqrt(fT) >= p.h) fT = std::nextafter(fT, 0.0f);
+        while (!(std::sqrt(fT) >= p.h)) fT = std::nextafter(fT, INFINITY);
+        const float ext = std::max({std::fabs(p.lo[0]), std::fabs(p.hi[0]), std::fabs(p.lo[1]),
+                                    std::fabs(p.hi[1]), p.h});
+        feps = 4e-6f * ext;
     }
 };
 
@@ -210,8 +230,8 @@ static void grid_sort(const K2<R>& c, Scene2<R>& S) {
     S.key.assign(n, 0);
     std::vector<int32_t> cnt(nc, 0), fill(nc, 0), lst(n);
     for (int i = 0; i < n; ++i) {
-        const int ix = cell_axis2(S.x[i], c.lox, c.H, c.nx);
-        const int iy = cell_axis2(S.y[i], c.loy, c.H, c.ny);
+        const int ix = cell_axis2(S.x[i], c.lox, c.hc, c.nx);
+        const int iy = cell_axis2(S.y[i], c.loy, c.hc, c.ny);
         S.key[i] = ix * c.ny + iy;
         cnt[S.key[i]]++;
     }
@@ -233,14 +253,36 @@ static void grid_sort(const K2<R>& c, Scene2<R>& S) {
     perm(S.en);
 }
 
-// The candidates of a query at (px, py): the 3x3 cells around its cell,
-// column (ix) outer, each column one contiguous slot range.
+// The candidates of a query at (px, py): the cell columns ix-2..ix+2 (cells
+// H/2) around its column, in order, each cut to the rows the disc of radius
+// H reaches (bounds widened by a margin, computed in float from the float
+// coordinates), each one contiguous slot range.  Every point within H of
+// the query lies in them, except a neighbour that its density pass moved
+// out of its cell (un-penetration): the cell lists hold the positions the
+// grid was built from, as the reference's do.  Pair terms accumulate only
+// the neighbours, in this order.
+template <class R>
+static inline bool column_rows(const K2<R>& c, float x, float y, int ix, int cx, int& ylo,
+                               int& yhi) {
+    float dx = 0.0f;
+    if (cx < ix) dx = x - (c.flox + (float)(cx + 1) * c.fhc);
+    else if (cx > ix) dx = (c.flox + (float)cx * c.fhc) - x;
+    dx = std::max(dx - c.feps, 0.0f);
+    const float d2 = dx * dx;
+    if (!(d2 < c.fT)) return false;
+    const float ry = std::sqrt(c.fT - d2) + c.feps;
+    ylo = cell_axis2(y - ry, c.floy, c.fhc, c.ny);
+    yhi = cell_axis2(y + ry, c.floy, c.fhc, c.ny);
+    return true;
+}
+
 template <class R, class F>
 static inline void for_candidates(const K2<R>& c, const Scene2<R>& S, R px, R py, F&& f) {
-    const int ix = cell_axis2(px, c.lox, c.H, c.nx);
-    const int iy = cell_axis2(py, c.loy, c.H, c.ny);
-    const int ylo = std::max(iy - 1, 0), yhi = std::min(iy + 1, c.ny - 1);
-    for (int cx = std::max(ix - 1, 0); cx <= std::min(ix + 1, c.nx - 1); ++cx) {
+    const float fx = (float)px, fy = (float)py;
+    const int ix = cell_axis2(fx, c.flox, c.fhc, c.nx);
+    for (int cx = std::max(ix - 2, 0); cx <= std::min(ix + 2, c.nx - 1); ++cx) {
+        int ylo, yhi;
+        if (!column_rows(c, fx, fy, ix, cx, ylo, yhi)) continue;
         const int b = S.offset[cx * c.ny + ylo], e = S.offset[cx * c.ny + yhi + 1];
         for (int j = b; j < e; ++j) f(j);
     }
@@ -414,10 +456,11 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
         S.vy.swap(nvy);
     }
     // ---- column flux of the last substep ----
-    S.colsum.assign(c.nx, R(0));
+    S.colsum.assign(c.nxc, R(0));
     if (c.band > R(0))
-        for (int cx = 0; cx < c.nx; ++cx)
-            S.colsum[cx] = column_sum(flux.data(), S.offset[cx * c.ny], S.offset[(cx + 1) * c.ny]);
+        for (int cx = 0; cx < c.nxc; ++cx)
+            S.colsum[cx] = column_sum(flux.data(), S.offset[2 * cx * c.ny],
+                                      S.offset[std::min(2 * cx + 2, c.nx) * c.ny]);
     // ---- wave substeps, free boundary (Wave1D_cs.glsl:120-157 or
     //      Shallow1D_cs.glsl:133-167) ----
     std::vector<R> a(S.h0), b(S.h1), o(L);
@@ -443,9 +486,9 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
             const R gx = (R(k) + R(0.5)) * cw * c.invH - R(0.5);
             const R f0 = std::floor(gx);
             const R al = gx - f0;
-            const R f = clampf_nan(f0, R(-2), R(c.nx + 1));
-            const int c0 = std::min(std::max((int)f, 0), c.nx - 1);
-            const int c1 = std::min(std::max((int)f + 1, 0), c.nx - 1);
+            const R f = clampf_nan(f0, R(-2), R(c.nxc + 1));
+            const int c0 = std::min(std::max((int)f, 0), c.nxc - 1);
+            const int c1 = std::min(std::max((int)f + 1, 0), c.nxc - 1);
             const R Sx = (R(1) - al) * S.colsum[c0] + al * S.colsum[c1];
             a[k] = a[k] + kd * Sx * c.invH;
         }

@@ -174,7 +174,7 @@ def test_sort_is_stable_counting_sort(o2):
     o.step(spec.dt, 1)
     nx, ny = o2.dims(spec.params)
     p = spec.params
-    H = np.float32(p.h)
+    H = np.float32(0.5) * np.float32(p.h)      # cells of H/2
     for s in range(spec.nscenes):
         g = o.grid(s)
         b = int(spec.pbeg()[s])
