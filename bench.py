@@ -175,30 +175,35 @@ def measure_fit(args) -> dict:
     (N_s log-uniform in [8k, 128k], heightfield sides uniform in
     [128, 1024], seed 4), 128 steps checkpointed every 16, all scenes'
     forward, loss and adjoint in one hipGraph replayed per iteration; six
-    scalars per scene fitted by Adam toward seeded target heightfields."""
+    scalars per scene fitted toward seeded target heightfields by Adam with
+    per-scene step control (a proposal that raises the loss is rejected and
+    the step halved), the attenuation as log(1 - atten)."""
     import numpy as np
 
     from coupledwateranimation_amd import fit
 
     t0 = time.perf_counter()
-    specs, targets, truth = fit.fit_workload(nscenes=args.fit_scenes)
+    specs, targets, factors = fit.fit_workload(nscenes=args.fit_scenes)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     run = fit.FitRun(specs, targets)
     t_capture = time.perf_counter() - t0
     res = run.descend(iters=args.fit_iters)
+    truth = fit.true_theta(run, factors)
+    err0 = fit.theta_errors(run.theta0, truth)
+    err1 = fit.theta_errors(res.best_theta, truth)
     run.close()
     steps = specs[0].steps
     n = np.array([sp.n for sp in specs])
     evals = np.array(res.eval_s[1:] if len(res.eval_s) > 1 else res.eval_s)
     t_eval = float(np.median(evals))
-    ratio = res.losses[-1] / np.maximum(res.losses[0], 1e-300)
-    best = res.best_loss / np.maximum(res.losses[0], 1e-300)
-    err0 = np.abs(1.0 - truth).mean(axis=1)
-    err1 = np.abs(res.best_u - truth).mean(axis=1)
+    ratio = res.accepted[-1] / np.maximum(res.accepted[0], 1e-300)
     per_scene = res.tape_bytes + res.scene_bytes
+    names = fit.THETA_NAMES
+    ident = [names.index("wave_speed"), names.index("atten")]
     print(f"bench: fit {len(specs)} scenes, {int(n.sum())} particles, {args.fit_iters} iterations: "
-          f"eval {1e3 * t_eval:.1f} ms, best/initial loss median {np.median(best):.3e}", flush=True)
+          f"eval {1e3 * t_eval:.1f} ms, final/initial loss median {np.median(ratio):.3e}, "
+          f"max {ratio.max():.3e}", flush=True)
     return {
         "workload": f"{len(specs)} 3D scenes, 8k-128k particles, 128^2-1024^2 heightfields, "
                     f"{steps} steps, checkpoint every 16",
@@ -206,12 +211,18 @@ def measure_fit(args) -> dict:
         "iterations": args.fit_iters, "setup_s": t_setup, "capture_s": t_capture,
         "eval_ms": 1e3 * t_eval,
         "fwd_adjoint_particle_steps_per_s": float(n.sum()) * steps / t_eval,
-        "loss_ratio_last": {"median": float(np.median(ratio)), "max": float(ratio.max()),
-                            "min": float(ratio.min())},
-        "loss_ratio_best": {"median": float(np.median(best)), "max": float(best.max()),
-                            "min": float(best.min())},
-        "param_error": {"initial_mean": float(err0.mean()), "best_mean": float(err1.mean()),
-                        "note": "mean |theta/theta0 - true factor| over the six scalars"},
+        "loss_ratio_final": {"median": float(np.median(ratio)), "max": float(ratio.max()),
+                             "min": float(ratio.min())},
+        "rejected_proposals": {"median": float(np.median(res.rejected)),
+                               "max": int(res.rejected.max())},
+        "param_error": {
+            "per_scalar_initial": {k: float(err0[:, i].mean()) for i, k in enumerate(names)},
+            "per_scalar_final": {k: float(err1[:, i].mean()) for i, k in enumerate(names)},
+            "identifiable_initial": float(err0[:, ident].mean()),
+            "identifiable_final": float(err1[:, ident].mean()),
+            "note": "mean over scenes of |theta / theta_true - 1| (atten: of 1 - atten); "
+                    "identifiable = wave_speed and atten, the scalars that shape the "
+                    "heightfield directly"},
         "hbm_bytes_per_scene": {"median": float(np.median(per_scene)),
                                 "max": float(per_scene.max()), "total": float(per_scene.sum())},
     }

@@ -69,7 +69,9 @@ def fit_workload(nscenes: int = 64, seed: int = 4, steps: int = 128, dt: float =
     differentiable dam-break physics (configs.dam_break_diff) at dt = 3e-4
     (inside the wave CFL limit at 1024 texels), `steps` steps.  Targets:
     the same scene run on the device with its six scalars scaled by
-    U[0.8, 1.2] (wave speed through lambda, attenuation capped at 0.9999).
+    U[0.8, 1.2] (wave speed through lambda; for the attenuation the factor
+    scales the per-step loss 1 - atten, so the target's wave keeps about the
+    initial guess's 0.999^128 = 0.88 of its amplitude and stays in view).
     Returns (specs, targets, true factors)."""
     rng = np.random.default_rng(seed)
     ns = np.round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), nscenes)))
@@ -81,8 +83,7 @@ def fit_workload(nscenes: int = 64, seed: int = 4, steps: int = 128, dt: float =
                                       dt=dt, steps=steps)
         specs.append(spec)
         ro = Rollout(spec, steps, 16)
-        th = np.array(ro.theta0(1.0)) * factors[s]
-        th[4] = min(th[4], 0.9999)
+        th = _scaled(np.array(ro.theta0(1.0)), factors[s])
         p = ro.params_for(th)
         with Scene(p, spec.pos, spec.vel, None, spec.wave_size, spec.height0,
                    spec.height1) as sc:
@@ -91,20 +92,43 @@ def fit_workload(nscenes: int = 64, seed: int = 4, steps: int = 128, dt: float =
     return specs, targets, factors
 
 
+def _scaled(theta: np.ndarray, factors: np.ndarray) -> np.ndarray:
+    th = theta * factors
+    th[..., ATTEN] = 1.0 - (1.0 - theta[..., ATTEN]) * factors[..., ATTEN]
+    return th
+
+
+# Fit coordinates: theta = theta0 * x for the multiplicative scalars (visc,
+# gas_const, rest_rho, wave_speed, couple_strength); the attenuation as
+# x = log(1 - atten), so the steep approach to atten -> 1 is a straight line
+# and the bound atten < 1 holds without a clamp inside the descent.
+ATTEN = 4
+X_LO = np.array([0.5, 0.5, 0.5, 0.5, np.log(1e-5), 0.5])
+X_HI = np.array([1.5, 1.5, 1.5, 1.5, np.log(0.5), 1.5])
+# Step scale per coordinate, relative to its range: the three particle
+# scalars (visc, gas_const, rest_rho) reach the heightfield only through the
+# particle flux of a chaotic particle system, so their gradients over 128
+# steps are weak and noisy; they move at a tenth of the wave scalars' pace.
+X_STEP = np.array([0.1, 0.1, 0.1, 1.0, 1.0, 1.0])
+
+
 @dataclass
 class FitResult:
-    losses: np.ndarray          # (iters + 1, nscenes): loss before each update, and final
-    u: np.ndarray               # (iters + 1, nscenes, 6): theta / theta0
+    losses: np.ndarray          # (iters + 1, nscenes): loss of each evaluated iterate
+    accepted: np.ndarray        # (iters + 1, nscenes): loss of the accepted iterate
+    x: np.ndarray               # (iters + 1, nscenes, 6): accepted fit coordinates
     eval_s: list                # wall time of each graph evaluation
     tape_bytes: np.ndarray      # per scene
     scene_bytes: np.ndarray     # per scene
-    best_loss: np.ndarray       # per scene: lowest loss evaluated
-    best_u: np.ndarray          # per scene: the u that reached it
+    best_loss: np.ndarray       # per scene: the final accepted loss (the lowest evaluated)
+    best_theta: np.ndarray      # per scene: the physical scalars that reached it
+    rejected: np.ndarray        # per scene: proposals rejected
 
 
 class FitRun:
     """Scenes at their initial guesses (theta0 of each spec), the fit graph,
-    and an Adam descent in u = theta / theta0."""
+    and a descent of Adam steps with per-scene step control in the fit
+    coordinates (see X_LO / X_HI)."""
 
     def __init__(self, specs, targets, ckpt_every: int = 16):
         import torch  # noqa: F401  (device memory accounting below)
@@ -125,45 +149,75 @@ class FitRun:
             raise ValueError("one dt and one step count for the whole batch")
         self.fit = FitBatch(self.scenes, targets, steps.pop(), ckpt_every, dts.pop())
 
-    def set_u(self, u: np.ndarray) -> None:
-        for s, (r, th0) in enumerate(zip(self.ro, self.theta0)):
-            th = th0 * u[s]
-            th[4] = min(th[4], 0.99999)
-            self.scenes[s].set_params(r.params_for(th))
+    def x0(self) -> np.ndarray:
+        """Fit coordinates of the initial guesses."""
+        x = np.ones((len(self.scenes), 6))
+        x[:, ATTEN] = np.log1p(-self.theta0[:, ATTEN])
+        return x
 
-    def evaluate(self, u: np.ndarray):
-        """Losses and d loss / d u (nscenes x 6) at u."""
-        self.set_u(u)
+    def theta_of(self, x: np.ndarray) -> np.ndarray:
+        th = self.theta0 * x
+        th[:, ATTEN] = -np.expm1(x[:, ATTEN])
+        return th
+
+    def evaluate(self, x: np.ndarray):
+        """Losses and d loss / d x (nscenes x 6) at fit coordinates x."""
+        th = self.theta_of(x)
+        for s, r in enumerate(self.ro):
+            self.scenes[s].set_params(r.params_for(th[s]))
         losses, grads = self.fit.run()
-        gu = np.array([r.chain_theta(g, th0 * uu) * th0
-                       for r, g, th0, uu in zip(self.ro, grads, self.theta0, u)])
-        return losses, gu
+        gth = np.array([r.chain_theta(g, t) for r, g, t in zip(self.ro, grads, th)])
+        gx = gth * self.theta0
+        gx[:, ATTEN] = gth[:, ATTEN] * -(1.0 - th[:, ATTEN])   # d atten / d log(1 - atten)
+        return losses, gx
 
-    def descend(self, iters: int = 50, lr: float = 0.01, beta1: float = 0.9,
-                beta2: float = 0.999) -> FitResult:
+    def descend(self, iters: int = 50, lr: float = 0.02, beta1: float = 0.9,
+                beta2: float = 0.999, grow: float = 1.25) -> FitResult:
+        """Adam per scene from the last accepted iterate: a proposal whose
+        loss is not below the accepted one is rejected and that scene's step
+        is halved (accepted steps grow it back by `grow`, up to `lr`), so the
+        accepted loss never rises.  One graph evaluation per iteration."""
         ns = len(self.scenes)
-        u = np.ones((ns, 6))
-        m = np.zeros_like(u)
-        v = np.zeros_like(u)
-        L_hist, u_hist, times = [], [u.copy()], []
-        for it in range(iters + 1):
+        x_acc = self.x0()
+        t0 = time.perf_counter()
+        loss_acc, g_acc = self.evaluate(x_acc)
+        times = [time.perf_counter() - t0]
+        m = np.zeros_like(x_acc)
+        v = np.zeros_like(x_acc)
+        k = np.zeros(ns)                       # accepted Adam steps per scene
+        step = np.full(ns, lr)
+        rejected = np.zeros(ns, np.int64)
+
+        def moments(g, mask):
+            m[mask] = beta1 * m[mask] + (1 - beta1) * g[mask]
+            v[mask] = beta2 * v[mask] + (1 - beta2) * g[mask] ** 2
+            k[mask] += 1
+
+        moments(g_acc, np.ones(ns, bool))
+        L_hist, A_hist, x_hist = [loss_acc.copy()], [loss_acc.copy()], [x_acc.copy()]
+        span = (X_HI - X_LO) * X_STEP
+        for _ in range(iters):
+            mh = m / (1 - beta1 ** k)[:, None]
+            vh = v / (1 - beta2 ** k)[:, None]
+            # steps are taken relative to each coordinate's range (the
+            # multiplicative scalars have range 1, log(1 - atten) about 11)
+            prop = np.clip(x_acc - step[:, None] * span * mh / (np.sqrt(vh) + 1e-12), X_LO, X_HI)
             t0 = time.perf_counter()
-            loss, g = self.evaluate(u)
+            loss, g = self.evaluate(prop)
             times.append(time.perf_counter() - t0)
+            ok = np.isfinite(loss) & (loss < loss_acc)
+            x_acc[ok], loss_acc[ok], g_acc[ok] = prop[ok], loss[ok], g[ok]
+            moments(g_acc, ok)
+            step[ok] = np.minimum(step[ok] * grow, lr)
+            step[~ok] *= 0.5
+            rejected += ~ok
             L_hist.append(loss)
-            if it == iters:
-                break
-            m = beta1 * m + (1 - beta1) * g
-            v = beta2 * v + (1 - beta2) * g * g
-            mh = m / (1 - beta1 ** (it + 1))
-            vh = v / (1 - beta2 ** (it + 1))
-            u = np.clip(u - lr * mh / (np.sqrt(vh) + 1e-12), 0.5, 1.5)
-            u_hist.append(u.copy())
-        Lh, uh = np.array(L_hist), np.array(u_hist)
-        best = np.argmin(Lh, axis=0)
-        return FitResult(Lh, uh, times, np.array([self.fit.tape_bytes(s) for s in range(ns)]),
-                         np.array(self.scene_bytes), Lh[best, np.arange(ns)],
-                         uh[best, np.arange(ns)])
+            A_hist.append(loss_acc.copy())
+            x_hist.append(x_acc.copy())
+        return FitResult(np.array(L_hist), np.array(A_hist), np.array(x_hist), times,
+                         np.array([self.fit.tape_bytes(s) for s in range(ns)]),
+                         np.array(self.scene_bytes), loss_acc.copy(), self.theta_of(x_acc),
+                         rejected)
 
     def close(self):
         self.fit.close()
@@ -175,6 +229,19 @@ class FitRun:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def true_theta(run: FitRun, factors: np.ndarray) -> np.ndarray:
+    """The physical scalars the targets were made with (fit_workload)."""
+    return _scaled(run.theta0, factors)
+
+
+def theta_errors(theta: np.ndarray, truth: np.ndarray) -> np.ndarray:
+    """Per scene and scalar: |theta / truth - 1| for the multiplicative
+    scalars, |(1 - atten) / (1 - atten_true) - 1| for the attenuation."""
+    e = np.abs(theta / truth - 1.0)
+    e[:, ATTEN] = np.abs((1.0 - theta[:, ATTEN]) / (1.0 - truth[:, ATTEN]) - 1.0)
+    return e
 
 
 def _free_bytes() -> int:

@@ -106,14 +106,23 @@ def test_fit_graph_reads_updated_parameters(gpu, cwa):
 
 
 def test_adam_descent_lowers_every_loss(gpu, cwa):
+    """The accepted loss of every scene never rises (rejected proposals
+    halve the step) and ends below the initial one; the fitted scalars are
+    the physical ones the scenes were last set to."""
     from coupledwateranimation_amd import fit
 
     specs, targets, truth = _small(nscenes=3, steps=24)
     with fit.FitRun(specs, targets, ckpt_every=8) as run:
         res = run.descend(iters=12, lr=0.05)
-    _say("losses", res.losses[0], "->", res.losses[-1])
-    assert np.all(np.isfinite(res.losses))
-    assert np.all(res.losses[-1] < res.losses[0])
+        truth_th = fit.true_theta(run, truth)
+        e0 = fit.theta_errors(run.theta0, truth_th)
+        e1 = fit.theta_errors(res.best_theta, truth_th)
+    _say("losses", res.accepted[0], "->", res.accepted[-1], "rejected", res.rejected)
+    _say("identifiable error", e0[:, [3, 4]].mean(), "->", e1[:, [3, 4]].mean())
+    assert np.all(np.isfinite(res.accepted))
+    assert np.all(np.diff(res.accepted, axis=0) <= 0.0)
+    assert np.all(res.accepted[-1] < res.accepted[0])
+    assert np.all(res.best_loss == res.accepted[-1])
     assert np.all(res.tape_bytes > 0) and np.all(res.scene_bytes > 0)
 
 
