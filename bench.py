@@ -97,9 +97,39 @@ KERNEL_KEYS = [
     ("cwa::b2d::k2_hash(", "batched", "hash"), ("cwa::b2d::k2_scatter(", "batched", "scatter"),
     ("cwa::b2d::k2_reorder(", "batched", "reorder"), ("cwa::b2d::k2_wave(", "batched", "wave"),
     ("cwa::k_density(", "plain", "density"), ("cwa::k_force_integrate(", "plain", "force_integrate"),
-    ("cwa::k_hash_count(", "plain", "hash"), ("cwa::k_scatter_tmp(", "plain", "scatter"),
+    ("cwa::k_hash_count(", "plain", "hash"), ("cwa::k_scatter_runs(", "plain", "scatter"),
     ("cwa::k_stable_reorder(", "plain", "reorder"), ("cwa::k_wave4(", "plain", "wave"),
 ]
+
+
+def kernel_sources_digest() -> str:
+    """sha256 over the HIP sources and headers of the library, in name
+    order: traffic.json records the digest of the code it was measured on,
+    and a bench run of other code reports no traffic figure."""
+    import hashlib
+
+    h = hashlib.sha256()
+    src = ROOT / "coupledwateranimation_amd" / "csrc"
+    for p in sorted(list(src.glob("*.hip")) + list(src.glob("*.hpp"))):
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
+    return h.hexdigest()
+
+
+def stored_traffic(workload: str, kernel: str):
+    """(bytes per launch, note) of `kernel` from profiles/traffic.json, or
+    (None, reason) when absent or measured on other kernel sources."""
+    tf = ROOT / "profiles" / "traffic.json"
+    if not tf.exists():
+        return None, "no profiles/traffic.json"
+    try:
+        d = json.loads(tf.read_text())
+        if d.get("_sources") != kernel_sources_digest():
+            return None, "profiles/traffic.json was measured on other kernel sources"
+        return d[workload][kernel]["bytes"], ("rocprofv3 FETCH_SIZE + WRITE_SIZE passes of "
+                                              "these kernel sources (profiles/traffic.json)")
+    except (ValueError, KeyError, TypeError):
+        return None, f"no entry for {kernel} in profiles/traffic.json"
 
 
 def write_traffic(args) -> int:
@@ -128,7 +158,8 @@ def write_traffic(args) -> int:
     plain = "dam_break_1048576_2048"
     batched = "batched 2D dam-break, 256 scenes"
     out = {"_calibration": {"kernel": cal[:80], "copy_kib": copy_kib,
-                            "fetch_factor": f_fetch, "write_factor": f_write}}
+                            "fetch_factor": f_fetch, "write_factor": f_write},
+           "_sources": kernel_sources_digest()}
     for name in fetch:
         for prefix, wl, key in KERNEL_KEYS:
             if name.startswith(prefix) or name.startswith("void " + prefix):
@@ -293,13 +324,7 @@ def measure_batched(args) -> dict:
                "sample": f"{args.batched_cpu_steps} step(s) of the same 256-scene batch from "
                          f"its initial state, fp32 oracle, OpenMP over scenes, {ct:.2f} s"}
         print(f"bench: batched CPU oracle {threads} threads: {ct:.2f} s", flush=True)
-    traffic = None
-    tf = ROOT / "profiles" / "traffic.json"
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text())[spec.name][dominant]["bytes"]
-        except (ValueError, KeyError, TypeError):
-            traffic = None
+    traffic, traffic_src = stored_traffic(spec.name, dominant)
     return {
         "workload": spec.name, "scenes": spec.nscenes, "particles": n, "wave_cells": nc,
         "steps": steps, "dt": spec.dt, "ms_per_step": ms_step,
@@ -310,7 +335,8 @@ def measure_batched(args) -> dict:
         "step_bytes": step_bytes, "step_gbs": step_bytes / (ms_step * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic, "alg_bytes": alg[dominant]},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "alg_bytes": alg[dominant]},
         "cpu_baseline": cpu, "finite": finite,
     }
 
@@ -460,13 +486,7 @@ def main() -> int:
     achieved = dom_bytes / (kernel_ms[dominant] * 1e-3) / 1e9 if dom_bytes else None
     step_bytes = sum(BYTES_PER_PARTICLE.values()) * n + BYTES_PER_CELL * ncell + \
         BYTES_PER_WAVE_CELL * W * H
-    traffic = None
-    tf = ROOT / "profiles" / "traffic.json"
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text())[spec.name][dominant]["bytes"]
-        except (ValueError, KeyError, TypeError):
-            traffic = None
+    traffic, traffic_src = stored_traffic(spec.name, dominant)
 
     copy = measure_copy(args.copy_mib) if args.copy_mib > 0 else None
     if copy:
@@ -521,7 +541,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
-                     "traffic": traffic, "alg_bytes": dom_bytes},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "alg_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "finite": finite,
         "state_after": state_stats,

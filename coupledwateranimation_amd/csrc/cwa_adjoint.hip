@@ -27,6 +27,7 @@
 
 #include "cwa_adjoint.hpp"
 #include "cwa_device.hpp"
+#include "cwa_kernels.hpp"
 
 namespace cwa {
 
@@ -554,11 +555,11 @@ __global__ void __launch_bounds__(256) k_adj_local(const DevParams* __restrict__
 // adjoint of particle k (EOS, rest-density clamp, wave term, flux).
 // G lanes per particle (consecutive lanes of a wave): each walks every G-th
 // neighbour, the group's sums are added in a fixed tree and its first lane
-// finishes the particle.  G = 4 for scenes of up to kAdjSplitMaxN
-// particles, which would otherwise keep one wave per SIMD.
+// finishes the particle.  G = 4 for scenes of up to lane_split_limit()
+// particles (taken when a tape is created), which would otherwise keep one
+// wave per SIMD.
 // ---------------------------------------------------------------------------
 constexpr int kAdjSplit = 4;
-constexpr int kAdjSplitMaxN = 1 << 18;
 
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
@@ -763,8 +764,8 @@ __global__ void __launch_bounds__(256) k_adj_rho_pairs(
 static inline int nblk(long long n, int t) { return (int)((n + t - 1) / t); }
 
 int adj_wave_blocks(const DevParams& P) { return ((P.W + 63) / 64) * ((P.H + 3) / 4); }
-static int adj_pair_group(int n) { return n <= kAdjSplitMaxN ? kAdjSplit : 1; }
-int adj_pair_blocks(int n) { return nblk((long long)n * adj_pair_group(n), 256); }
+int adj_pair_group(int n) { return n <= lane_split_limit() ? kAdjSplit : 1; }
+int adj_pair_blocks(int n, int group) { return nblk((long long)n * group, 256); }
 
 // Loss seed of a heightfield fit: L = mean((h - t)^2), dL/dh = 2 (h - t) / N.
 // Per-block partial sums (double, fixed order: grid-stride per thread, then
@@ -845,14 +846,15 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
-                            hipStream_t st) {
+                            int group, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (adj_pair_group(n) == kAdjSplit)
-        k_adj_pairs<kAdjSplit><<<adj_pair_blocks(n), 256, 0, st>>>(
+    if (group != 1 && group != kAdjSplit) return hipErrorInvalidValue;
+    if (group == kAdjSplit)
+        k_adj_pairs<kAdjSplit><<<adj_pair_blocks(n, group), 256, 0, st>>>(
             P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w, pw, rho_raw, H0, Sbar, adjrec,
             rhob_d, G0n, xb_loc, vb_loc, rtb, acc);
     else
-        k_adj_pairs<1><<<adj_pair_blocks(n), 256, 0, st>>>(
+        k_adj_pairs<1><<<adj_pair_blocks(n, group), 256, 0, st>>>(
             P.dev, dt, n, pos, vel, offset, nbw, nrec, rho_w, pw, rho_raw, H0, Sbar, adjrec,
             rhob_d, G0n, xb_loc, vb_loc, rtb, acc);
     return hipGetLastError();

@@ -23,7 +23,8 @@ enum GradSlot {
 // Parameter partials are accumulated per block (fp64, no atomics) into acc
 // arrays of blocks x slots; launch_reduce_partials sums them in block order.
 int adj_wave_blocks(const DevParams& P);  // blocks of launch_adj_wave (5 slots each)
-int adj_pair_blocks(int n);               // blocks of launch_adj_pairs (3 slots each)
+int adj_pair_group(int n);                // lanes per particle of the pair adjoint now
+int adj_pair_blocks(int n, int group);    // blocks of launch_adj_pairs (3 slots each)
 // MSE of a heightfield against a target: seed G = 2 (h - t)/N, loss = mean.
 hipError_t launch_mse_seed(const float* h, const float* target, int N, float* G, double* part,
                            int nblocks, double* loss, hipStream_t st);
@@ -44,7 +45,7 @@ hipError_t launch_adj_pairs(const DevParams& P, float dt, int n, const float4* p
                             const float* rho_raw, const float* H0, const float* Sbar,
                             const float4* adjrec, const float* rhob_d, float* G0n,
                             float4* xb_loc, float4* vb_loc, float* rtb, double* acc,
-                            hipStream_t st);
+                            int group, hipStream_t st);
 hipError_t launch_adj_rho_pairs(const DevParams& P, int n, const float4* pos, const int* offset,
                                 const uint2* nbw, const float* rtb, const float4* xb_loc,
                                 const float4* vb_loc, const int* list, float4* xb_out,

@@ -21,11 +21,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "cwa_device.hpp"
 #include "cwa_kernels.hpp"
 
 namespace cwa {
+
+static std::atomic<long long> g_lane_split{kLaneSplitDefault};
+long long lane_split_limit() { return g_lane_split.load(std::memory_order_relaxed); }
+long long set_lane_split_limit(long long max_n) { return g_lane_split.exchange(max_n); }
 
 // ---------------------------------------------------------------------------
 // utilities
@@ -545,14 +550,13 @@ __global__ void __launch_bounds__(256) k_density(const DevParams* __restrict__ P
     density_finish(P, i, pi, vi, acc, H0, rho_w, pw, flux, nrec, rho_raw);
 }
 
-// Density for scenes of up to kDensitySplitMaxN particles: kDensitySplit
+// Density for scenes of up to lane_split_limit() particles: kDensitySplit
 // consecutive lanes per particle, each testing every kDensitySplit-th
 // candidate of each column, so a scene of some ten thousand particles still
 // keeps several waves per SIMD busy.  The group's neighbour bits are OR-ed
 // together (the words equal k_density's), its partial sums added in a fixed
 // tree, and the group's first lane finishes the particle.
 constexpr int kDensitySplit = 4;
-constexpr int kDensitySplitMaxN = 1 << 18;
 __global__ void __launch_bounds__(256) k_density_split(
     const DevParams* __restrict__ Pd, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const int* __restrict__ list,
@@ -1100,7 +1104,7 @@ hipError_t launch_density(const DevParams& P, const float4* bpos, const float4* 
                           float* rho_w, float2* pw, float* flux, uint2* nbw, float4* nrec,
                           float* rho_raw, float4* vel_out, float* vpad, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (n <= kDensitySplitMaxN)
+    if (n <= lane_split_limit())
         k_density_split<<<blocks_for((long long)n * kDensitySplit, 256), 256, 0, st>>>(
             P.dev, bpos, state_vel, list, offset, n, H0, rho_w, pw, flux, nbw, nrec, rho_raw,
             vel_out, vpad);

@@ -60,6 +60,13 @@ hipError_t launch_box_pairs(const float4* x, const float4* hw, int n, const floa
 // rho_raw (density before the wave term and the rest-density clamp) and
 // vel_out (the sorted velocities) are optional outputs for the adjoint tape
 // (null: not written).
+// Scenes of up to this many particles run the density pass (and a tape's
+// pair adjoint) with four lanes per particle, larger ones with one lane;
+// read at each density launch and at tape creation.  Default kLaneSplitDefault.
+constexpr long long kLaneSplitDefault = 1 << 18;
+long long lane_split_limit();
+long long set_lane_split_limit(long long max_n);  // returns the previous limit
+
 hipError_t launch_density(const DevParams& P, const float4* bpos, const float4* state_vel,
                           const int* list, const int* offset, int n, const float* H0,
                           float* rho_w, float2* pw, float* flux, uint2* nbw, float4* nrec,

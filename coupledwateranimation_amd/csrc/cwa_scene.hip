@@ -765,6 +765,12 @@ int cwa_device_count(void) {
 
 const char* cwa_kernel_name(int k) { return kernel_slot_name(k); }
 
+int cwa_set_lane_split_limit(long long max_n, long long* previous) {
+    const long long old = set_lane_split_limit(max_n < 0 ? kLaneSplitDefault : max_n);
+    if (previous) *previous = old;
+    return CWA_OK;
+}
+
 void cwa_params_reference(cwa_params* p) {
     if (!p) return;
     std::memset(p, 0, sizeof(*p));

@@ -91,6 +91,7 @@ class Tape {
     unsigned long long gen_;
     int nsteps_, every_, nseg_;
     int n_, W_, H_, ncell_, ncol_;
+    int group_ = 1;  // lanes per particle of the pair adjoint, fixed at creation
     float dt_ = 0.0f;
     bool forward_done_ = false;
     std::vector<Ckpt> ck_;
@@ -141,7 +142,8 @@ Tape::Tape(CoupledScene& s, int nsteps, int every, bool pipelined)
     Sbar_.resize(ncol_);
     pgrad_.resize(GRAD_NUM);
     acc_wave_.resize((size_t)adj_wave_blocks(s.devparams()) * 5);
-    acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1)) * 3);
+    group_ = adj_pair_group(n_);
+    acc_pair_.resize((size_t)adj_pair_blocks(std::max(n_, 1), group_) * 3);
     if (pipelined) {
         if (nseg_ > 1) {
             tp2_.resize(tp_.size());
@@ -335,7 +337,7 @@ void Tape::adjoint_step(const StepTape& t, hipStream_t st) {
                                    t.nbw.get(), t.nrec.get(), t.rho.get(), t.pw.get(),
                                    t.rho_raw.get(), t.h0, Sbar_.get(), adjrec_.get(),
                                    rhob_.get(), G0n_.get(), xl_.get(), vl_.get(), rtb_.get(),
-                                   acc_pair_.get(), st),
+                                   acc_pair_.get(), group_, st),
                   "adj pairs");
         hip_check(launch_adj_rho_pairs(P, n_, t.pos.get(), t.offset.get(), t.nbw.get(),
                                        rtb_.get(), xl_.get(), vl_.get(), t.list.get(),

@@ -30,6 +30,16 @@ def reference_params() -> L.Params:
     return p
 
 
+def set_lane_split_limit(max_n: int) -> int:
+    """cwa_set_lane_split_limit: scenes of up to max_n particles run the
+    density pass (and the pair adjoint of tapes created afterwards) with four
+    lanes per particle, larger ones with one; max_n < 0 restores the default
+    (2^18).  Returns the previous limit."""
+    prev = C.c_longlong()
+    L.check(L.lib().cwa_set_lane_split_limit(int(max_n), C.byref(prev)))
+    return int(prev.value)
+
+
 def _f32(a, shape=None):
     if a is None:
         return None

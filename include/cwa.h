@@ -134,7 +134,12 @@ typedef struct cwa_init {
     const float* height1;   /* w*h floats, previous field h^(n-1), or NULL = height0 */
 } cwa_init;
 
-/* Device pointers of the particle buffers (replaces particles_ssbo). */
+/* Device pointers of the particle buffers (replaces particles_ssbo): the
+ * newest state, in the cell order of the last step.  The library alternates
+ * two state buffers, so the pointers hold until the next step, reinit or
+ * destroy.  Read-only: the last step already hashed these positions into
+ * the next step's cell keys, so a position written through the view would
+ * be sorted into a stale cell; change the state with cwa_scene_reinit. */
 typedef struct cwa_particle_view {
     int32_t n;
     float* pos;    /* float4[n] */
@@ -192,6 +197,17 @@ const char* cwa_last_error(void);
 /* Name of kernel slot k in cwa_kernel_times (e.g. "density"). */
 const char* cwa_kernel_name(int k);
 
+/* Lanes per particle (no reference counterpart: a tuning knob of this
+ * implementation, also used by tests to reach both kernel forms on small
+ * scenes).  Scenes of up to max_n particles run the density pass with four
+ * lanes per particle and the pair adjoint of a tape created afterwards with
+ * four; larger scenes one lane each.  Default 262144 (2^18); max_n < 0
+ * restores it.  Process-wide; taken at each density launch (a step graph
+ * keeps the form it was captured with) and at tape creation.  *previous
+ * (optional) receives the limit before the call.  Both forms find the same
+ * neighbour words; the density sums differ in summation order only. */
+int cwa_set_lane_split_limit(long long max_n, long long* previous);
+
 /* Reference defaults (Main.cpp:184-204 and the shader #defines) for a grid
  * over the reference box; the heightfield spans [0,0.5]^2 in xz. */
 void cwa_params_reference(cwa_params* p);
@@ -230,8 +246,9 @@ int cwa_scene_step(cwa_scene* s, float dt, int nsteps);
 /* Same as cwa_scene_step with HIP events around every kernel; `out`
  * receives the per-kernel average device times. */
 int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times* out);
-/* Replay-based execution: capture a 6-step cycle (the period of the grid
- * counter double buffer x the wave triple buffer) into a hipGraph and
+/* Replay-based execution: capture a 6-step cycle (the period of the double
+ * buffers of the grid counters / keys and of the particle state x the wave
+ * triple buffer) into a hipGraph and
  * launch it; nsteps that are not a multiple of 6 run the tail eagerly.
  * The graph is re-captured when dt changes. */
 int cwa_scene_step_graph(cwa_scene* s, float dt, int nsteps);

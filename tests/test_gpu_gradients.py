@@ -261,3 +261,57 @@ def test_tape_and_fit_refuse_a_changed_scene(gpu, cwa):
     fb.close()
     for s in scenes:
         s.close()
+
+
+def test_pair_adjoint_lane_forms_agree(gpu, cwa, oracle):
+    """The two forms of the density pass and the pair adjoint, four lanes
+    per particle (scenes up to 2^18 particles) and one lane (larger scenes),
+    forced on one small scene: a 2 048-particle dam break beside a
+    compressed block (spacing h/2.8, up to 27 particles per cell, 90
+    neighbours), whose long columns send particles through the re-search
+    by distance in both forms.  Seeds on every output of 3 steps: parameter
+    gradients and the initial-state gradients agree to 1e-4 (the forms sum
+    in different orders)."""
+    from coupledwateranimation_amd import configs
+    from coupledwateranimation_amd.gradient import Tape
+    from coupledwateranimation_amd.scene import Scene, set_lane_split_limit
+
+    spec = configs.dam_break_diff(n=2048, wave=32, dt=1e-3, steps=3)
+    p = spec.params
+    sp = p.h / 2.8
+    block = configs.lattice(12 ** 3, (0.55, 0.1, 0.55),
+                            (0.55 + 12 * sp, 0.1 + 12 * sp, 0.55 + 12 * sp), sp, 0.5, 0.1,
+                            (2, 0, 1), 11)
+    pos = np.concatenate([spec.pos, block])
+    n = pos.shape[0]
+    rng = np.random.default_rng(7)
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(scale=0.2, size=(n, 3))
+    g = oracle.grid_build(p, pos)
+    assert p.grid_ydiv == 1 and g["count"].max() * 3 > 64  # some column past 64 candidates
+    W, H = spec.wave_size
+    a, b = rng.normal(size=(H, W)), rng.normal(size=(H, W))
+    c, d = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+
+    def run(limit):
+        prev = set_lane_split_limit(limit)
+        try:
+            sc = Scene(p, pos, vel, None, (W, H), spec.height0, spec.height1)
+            tape = Tape(sc, spec.steps, 2)
+            tape.forward(spec.dt)
+            out = tape.backward(dheight0=a, dheight1=b, dpos=c, dvel=d)
+            tape.close()
+            sc.close()
+        finally:
+            set_lane_split_limit(prev)
+        return out
+
+    g4, dp4, dv4 = run(-1)
+    g1, dp1, dv1 = run(0)
+    for name in SCALARS:
+        _say(f"{name}: four lanes {g4[name]:.8e} one lane {g1[name]:.8e}")
+        assert _rel(g1[name], g4[name]) < 1e-4, name
+    for which, x1, x4 in (("pos", dp1, dp4), ("vel", dv1, dv4)):
+        e = np.abs(x1 - x4).max() / np.abs(x4).max()
+        _say(f"d{which}0: max difference {e:.3e} of the largest")
+        assert e < 1e-4, which

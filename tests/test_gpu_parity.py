@@ -443,12 +443,13 @@ def _dense_block(ratio):
 
 @pytest.mark.parametrize("name", ["dam_break_4k", "reference", "dam_break_4k_ydiv4",
                                   "dense_2.2", "dense_2.8"])
-def test_neighbour_counts_bit_exact(gpu, cwa, oracle, name):
+def test_neighbour_counts_bit_exact(gpu, cwa, oracle, lanes, name):
     """Neighbour count of every particle (self included) as the first
     step's density pass found it, against the oracle's grid search on the
     same positions; for the 4 096-particle scene also against brute-force
     all pairs.  The dense blocks drive the second words, long columns and
-    the re-search by distance."""
+    the re-search by distance.  Both density forms: four lanes per particle
+    and the one-lane pass of scenes above 2^18 particles."""
     if name.startswith("dense"):
         base, pos = _dense_block(float(name.split("_")[1]))
         params, W = base.params, 32
@@ -458,7 +459,7 @@ def test_neighbour_counts_bit_exact(gpu, cwa, oracle, name):
         spec = _specs(cwa)[name]
         params, pos, dt = spec.params, spec.pos, spec.dt
         s = spec.make_scene()
-    _say(f"neighbour counts {name}: {pos.shape[0]} particles")
+    _say(f"neighbour counts {name}: {pos.shape[0]} particles, {lanes} lane(s) per particle")
     with pytest.raises(cwa.CwaError) as e:
         s.neighbour_counts()          # no step yet
     assert e.value.code == -4
@@ -476,20 +477,21 @@ def test_neighbour_counts_bit_exact(gpu, cwa, oracle, name):
 
 
 @pytest.mark.parametrize("ratio", [2.2, 2.8])
-def test_dense_block_neighbour_paths(gpu, cwa, oracle, ratio):
+def test_dense_block_neighbour_paths(gpu, cwa, oracle, lanes, ratio):
     """A compressed block (spacing h/2.2 and h/2.8: ~10 and ~22 particles per
     cell, 50-90 neighbours each) drives the rarer neighbour-record paths of
     the density / force passes: second words for candidates 32-63 and
     columns past 64 candidates, whose particles re-search their neighbours
-    by distance.  One step against the
-    oracle, tolerances as the one-step test; the grid is bit-exact."""
+    by distance.  One step against the oracle, tolerances as the one-step
+    test, with both density forms (four lanes / one lane per particle); the
+    grid is bit-exact."""
     base, pos = _dense_block(ratio)
     n = pos.shape[0]
     vel = np.zeros_like(pos)
     vel[:, 1] = -0.2
     W = H = 32
     h0 = np.zeros((H, W), np.float32)
-    _say(f"dense block ratio {ratio}: {n} particles")
+    _say(f"dense block ratio {ratio}: {n} particles, {lanes} lane(s) per particle")
     s = cwa.Scene(base.params, pos, vel, None, (W, H), h0, h0)
     s.step(base.dt, 1)
     st = s.readback()
