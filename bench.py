@@ -206,9 +206,9 @@ def measure_fit(args) -> dict:
     (N_s log-uniform in [8k, 128k], heightfield sides uniform in
     [128, 1024], seed 4), 128 steps checkpointed every 16, all scenes'
     forward, loss and adjoint in one hipGraph replayed per iteration; six
-    scalars per scene fitted toward seeded target heightfields by Adam with
-    per-scene step control (a proposal that raises the loss is rejected and
-    the step halved), the attenuation as log(1 - atten)."""
+    scalars per scene fitted toward seeded target heightfields by L-BFGS
+    with per-scene step control (a proposal that does not lower the loss is
+    rejected and the step halved), the attenuation as log(1 - atten)."""
     import numpy as np
 
     from coupledwateranimation_amd import fit

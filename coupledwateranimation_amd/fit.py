@@ -5,7 +5,7 @@ all scenes' checkpointed forward passes, losses and adjoint sweeps.
 
     specs, targets, truth = fit.fit_workload(nscenes=64)   # seeded targets
     with fit.FitRun(specs, targets) as run:
-        result = run.descend(iters=50)
+        result = run.descend(iters=50)      # L-BFGS per scene
 """
 from __future__ import annotations
 
@@ -105,11 +105,9 @@ def _scaled(theta: np.ndarray, factors: np.ndarray) -> np.ndarray:
 ATTEN = 4
 X_LO = np.array([0.5, 0.5, 0.5, 0.5, np.log(1e-5), 0.5])
 X_HI = np.array([1.5, 1.5, 1.5, 1.5, np.log(0.5), 1.5])
-# Step scale per coordinate, relative to its range: the three particle
-# scalars (visc, gas_const, rest_rho) reach the heightfield only through the
-# particle flux of a chaotic particle system, so their gradients over 128
-# steps are weak and noisy; they move at a tenth of the wave scalars' pace.
-X_STEP = np.array([0.1, 0.1, 0.1, 1.0, 1.0, 1.0])
+# Scale of each fit coordinate in the descent: the spread of the scalars
+# around their guesses (a fifth; for log(1 - atten) a fifth too).
+X_SCALE = np.full(6, 0.2)
 
 
 @dataclass
@@ -127,8 +125,8 @@ class FitResult:
 
 class FitRun:
     """Scenes at their initial guesses (theta0 of each spec), the fit graph,
-    and a descent of Adam steps with per-scene step control in the fit
-    coordinates (see X_LO / X_HI)."""
+    and a quasi-Newton descent with per-scene step control in the fit
+    coordinates (see X_LO / X_HI / X_SCALE)."""
 
     def __init__(self, specs, targets, ckpt_every: int = 16):
         import torch  # noqa: F401  (device memory accounting below)
@@ -171,53 +169,74 @@ class FitRun:
         gx[:, ATTEN] = gth[:, ATTEN] * -(1.0 - th[:, ATTEN])   # d atten / d log(1 - atten)
         return losses, gx
 
-    def descend(self, iters: int = 50, lr: float = 0.02, beta1: float = 0.9,
-                beta2: float = 0.999, grow: float = 1.25) -> FitResult:
-        """Adam per scene from the last accepted iterate: a proposal whose
-        loss is not below the accepted one is rejected and that scene's step
-        is halved (accepted steps grow it back by `grow`, up to `lr`), so the
-        accepted loss never rises.  One graph evaluation per iteration."""
+    def descend(self, iters: int = 50, memory: int = 5, first_step: float = 0.25,
+                scale=None) -> FitResult:
+        """Limited-memory BFGS per scene in scaled fit coordinates z = (x -
+        x0) / scale, one graph evaluation per iteration.  The first direction
+        is the gradient, scaled so its largest component moves `first_step`;
+        later ones come from the scene's last `memory` accepted (s, y) pairs
+        (pairs with s.y <= 0 are skipped).  A proposal whose loss is not
+        below the accepted one is rejected and the step along the same
+        direction halved; an accepted one resets the step to 1 and takes a
+        new direction, so the accepted loss never rises."""
         ns = len(self.scenes)
-        x_acc = self.x0()
+        sc = np.asarray(X_SCALE if scale is None else scale, np.float64)
+        x0 = self.x0()
+        z = np.zeros_like(x0)
         t0 = time.perf_counter()
-        loss_acc, g_acc = self.evaluate(x_acc)
+        f, gx = self.evaluate(x0)
         times = [time.perf_counter() - t0]
-        m = np.zeros_like(x_acc)
-        v = np.zeros_like(x_acc)
-        k = np.zeros(ns)                       # accepted Adam steps per scene
-        step = np.full(ns, lr)
+        g = gx * sc
+        hist = [[] for _ in range(ns)]
+        alpha = np.ones(ns)
         rejected = np.zeros(ns, np.int64)
 
-        def moments(g, mask):
-            m[mask] = beta1 * m[mask] + (1 - beta1) * g[mask]
-            v[mask] = beta2 * v[mask] + (1 - beta2) * g[mask] ** 2
-            k[mask] += 1
+        def direction(k):
+            q = -g[k].copy()
+            pairs = hist[k]
+            a = []
+            for s_, y_ in reversed(pairs):
+                rho = 1.0 / float(y_ @ s_)
+                ai = rho * float(s_ @ q)
+                q -= ai * y_
+                a.append((rho, ai))
+            if pairs:
+                s_, y_ = pairs[-1]
+                q *= float(s_ @ y_) / float(y_ @ y_)
+            else:
+                q *= first_step / max(np.abs(q).max(), 1e-300)
+            for (s_, y_), (rho, ai) in zip(pairs, reversed(a)):
+                b = rho * float(y_ @ q)
+                q += (ai - b) * s_
+            return q
 
-        moments(g_acc, np.ones(ns, bool))
-        L_hist, A_hist, x_hist = [loss_acc.copy()], [loss_acc.copy()], [x_acc.copy()]
-        span = (X_HI - X_LO) * X_STEP
+        d = np.array([direction(k) for k in range(ns)])
+        lo, hi = (X_LO - x0) / sc, (X_HI - x0) / sc
+        L_hist, A_hist, x_hist = [f.copy()], [f.copy()], [x0.copy()]
         for _ in range(iters):
-            mh = m / (1 - beta1 ** k)[:, None]
-            vh = v / (1 - beta2 ** k)[:, None]
-            # steps are taken relative to each coordinate's range (the
-            # multiplicative scalars have range 1, log(1 - atten) about 11)
-            prop = np.clip(x_acc - step[:, None] * span * mh / (np.sqrt(vh) + 1e-12), X_LO, X_HI)
-            t0 = time.perf_counter()
-            loss, g = self.evaluate(prop)
-            times.append(time.perf_counter() - t0)
-            ok = np.isfinite(loss) & (loss < loss_acc)
-            x_acc[ok], loss_acc[ok], g_acc[ok] = prop[ok], loss[ok], g[ok]
-            moments(g_acc, ok)
-            step[ok] = np.minimum(step[ok] * grow, lr)
-            step[~ok] *= 0.5
+            zp = np.clip(z + alpha[:, None] * d, lo, hi)
+            t1 = time.perf_counter()
+            fp, gpx = self.evaluate(x0 + zp * sc)
+            times.append(time.perf_counter() - t1)
+            gp = gpx * sc
+            ok = np.isfinite(fp) & (fp < f)
+            for k in np.nonzero(ok)[0]:
+                s_, y_ = zp[k] - z[k], gp[k] - g[k]
+                if float(s_ @ y_) > 1e-12 * np.linalg.norm(s_) * np.linalg.norm(y_):
+                    hist[k].append((s_, y_))
+                    del hist[k][:-memory]
+                z[k], f[k], g[k] = zp[k], fp[k], gp[k]
+                alpha[k] = 1.0
+                d[k] = direction(k)
+            alpha[~ok] *= 0.5
             rejected += ~ok
-            L_hist.append(loss)
-            A_hist.append(loss_acc.copy())
-            x_hist.append(x_acc.copy())
+            L_hist.append(fp)
+            A_hist.append(f.copy())
+            x_hist.append(x0 + z * sc)
+        x = x0 + z * sc
         return FitResult(np.array(L_hist), np.array(A_hist), np.array(x_hist), times,
                          np.array([self.fit.tape_bytes(s) for s in range(ns)]),
-                         np.array(self.scene_bytes), loss_acc.copy(), self.theta_of(x_acc),
-                         rejected)
+                         np.array(self.scene_bytes), f.copy(), self.theta_of(x), rejected)
 
     def close(self):
         self.fit.close()
