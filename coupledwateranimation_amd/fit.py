@@ -219,7 +219,7 @@ class FitRun:
             fp, gpx = self.evaluate(x0 + zp * sc)
             times.append(time.perf_counter() - t1)
             gp = gpx * sc
-            ok = np.isfinite(fp) & (fp < f)
+            ok = np.isfinite(fp) & (fp < f) & np.all(np.isfinite(gp), axis=1)
             for k in np.nonzero(ok)[0]:
                 s_, y_ = zp[k] - z[k], gp[k] - g[k]
                 if float(s_ @ y_) > 1e-12 * np.linalg.norm(s_) * np.linalg.norm(y_):
