@@ -341,6 +341,64 @@ def measure_batched(args) -> dict:
     }
 
 
+def search_stats(p, pos, nb) -> dict:
+    """Neighbour-search figures of the state `pos` (n x 4, any order) on
+    the grid of `p`: candidates per particle of the density pass's column
+    search (numpy float32 restatement of its trimmed column ranges, without
+    the float margin), the per-wave cost (for 64 consecutive cell-sorted
+    particles, the longest candidate range of each of the 9 columns, summed:
+    what one wave's loop runs), and the neighbours per particle the device
+    found (`nb`, self included)."""
+    import numpy as np
+
+    f32 = np.float32
+    dims = [int(d) for d in p.grid_dims]
+    gmin = [f32(g) for g in p.grid_min]
+    cell = f32(p.cell_size)
+    cy = f32(cell / f32(p.grid_ydiv))
+    T = f32(p.h) * f32(p.h)
+
+    def axis(v, g, c, d):
+        return np.clip(np.floor((v - g) / c), 0, d - 1).astype(np.int64)
+
+    x, y, z = (pos[:, a].astype(f32) for a in range(3))
+    ix, iy, iz = axis(x, gmin[0], cell, dims[0]), axis(y, gmin[1], cy, dims[1]), \
+        axis(z, gmin[2], cell, dims[2])
+    key = (ix * dims[2] + iz) * dims[1] + iy
+    order = np.argsort(key, kind="stable")
+    offset = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=dims[0] * dims[1] *
+                                                        dims[2]))])
+    x, y, z, ix, iz = x[order], y[order], z[order], ix[order], iz[order]
+    n = len(x)
+    total = np.zeros(n, np.int64)
+    pad = (-n) % 64
+    wave_cost = 0
+    for dxc in (-1, 0, 1):
+        for dzc in (-1, 0, 1):
+            cx, cz = ix + dxc, iz + dzc
+            ok = (cx >= 0) & (cx < dims[0]) & (cz >= 0) & (cz < dims[2])
+            dx = np.where(cx < ix, x - (gmin[0] + (cx + 1).astype(f32) * cell),
+                          np.where(cx > ix, (gmin[0] + cx.astype(f32) * cell) - x, f32(0)))
+            dz = np.where(cz < iz, z - (gmin[2] + (cz + 1).astype(f32) * cell),
+                          np.where(cz > iz, (gmin[2] + cz.astype(f32) * cell) - z, f32(0)))
+            d2 = dx * dx + dz * dz
+            ok &= d2 < T
+            ry = np.sqrt(np.maximum(T - d2, f32(0)))
+            ylo, yhi = axis(y - ry, gmin[1], cy, dims[1]), axis(y + ry, gmin[1], cy, dims[1])
+            cxc, czc = np.clip(cx, 0, dims[0] - 1), np.clip(cz, 0, dims[2] - 1)
+            base = (cxc * dims[2] + czc) * dims[1]
+            ln = np.where(ok, offset[base + yhi + 1] - offset[base + ylo], 0)
+            total += ln
+            wave_cost += int(np.concatenate([ln, np.zeros(pad, np.int64)]).reshape(-1, 64)
+                             .max(axis=1).sum())
+    return {"candidates_per_particle": float(total.mean()),
+            "wave_loop_per_64": wave_cost / ((n + pad) / 64),
+            "neighbours_per_particle": {"mean": float(nb.mean()), "max": int(nb.max())},
+            "note": "after the timed steps; candidates from a host restatement of the "
+                    "density pass's column search, neighbours (self included) as the "
+                    "device found them"}
+
+
 def cpu_info() -> dict:
     """Host cores this process may use and the CPU model (/proc/cpuinfo)."""
     model = None
@@ -466,6 +524,10 @@ def main() -> int:
                    "rho_mean": float(st["rho"].mean()),
                    "vmax": float(np.abs(st["vel"][:, :3]).max()),
                    "hmax": float(np.abs(st["height0"]).max())}
+    search = search_stats(p, st["pos"], scene.neighbour_counts())
+    print(f"bench: search: {search['candidates_per_particle']:.1f} candidates / particle, "
+          f"{search['wave_loop_per_64']:.1f} per wave, neighbours "
+          f"{search['neighbours_per_particle']}", flush=True)
 
     # Per-kernel device time from HIP events on the scene's stream, over the
     # same trajectory: back to the initial state, the same warm-up, then the
@@ -546,6 +608,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "finite": finite,
         "state_after": state_stats,
+        "search": search,
         "adjoint": adjoint,
         "batched": batched,
         "fit": fitres,
