@@ -725,10 +725,11 @@ __global__ void __launch_bounds__(256) k_force_integrate(
             // kForceBatch neighbours per iteration: every record is loaded
             // before any pair is evaluated (slots past the end of the list hold
             // a zero record and are evaluated with zero weight); pairs are
-            // accumulated in order.  At four blocks per CU, 6 and 7 in flight
-            // measured fastest (MI355X, 1M-particle dam break: 4, 5, 6, 7, 8 ->
-            // 107, 105, 105, 104, 117 us).
-            constexpr int kForceBatch = 6;
+            // accumulated in order.  At four blocks per CU, with the particle's
+            // own record and previous force also in flight, 4 measured fastest
+            // (MI355X, 1M-particle dam break, same box: 2, 3, 4, 5, 6, 7, 8 ->
+            // 116.6, 112.3, 110.1, 115.2, 115.1, 116.5, 115.9 us).
+            constexpr int kForceBatch = 4;
             const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
             NbCursor nb(st, 256, nw);
             for (;;) {
