@@ -124,7 +124,7 @@ __device__ __forceinline__ void column_range(const DevParams& P, const int* __re
 // in one buffer of nb_buffer_uint2(n) uint2 (the counts follow the words).
 // Only a particle's non-empty words are written and read back.
 // ---------------------------------------------------------------------------
-constexpr int kNbEntries = 18;
+constexpr int kNbEntries = 16;
 constexpr int kNbMaxColumn = 64;
 
 __host__ __device__ constexpr size_t nb_buffer_uint2(size_t n) {
@@ -173,9 +173,10 @@ struct NbWriter {
 // order by NbCursor: one flat loop over all columns, so a wave runs as many
 // iterations as its busiest lane has neighbours rather than the sum over
 // columns of each column's busiest lane.  Room for every word also caps a
-// 256-thread block at 36 KB of LDS, four blocks per CU, which measured
-// faster for the force pass than six (fewer waves gathering neighbour
-// records through one L1).
+// 256-thread block at 32 KB of LDS, five blocks per CU (the force pass
+// measured 109.4 us with 16 words a particle, five blocks per CU; 110.6 with
+// 18, four blocks; 113.2 with 12, six blocks: more waves gathering neighbour
+// records through one L1 cost more than they hide).
 constexpr int kNbStage = kNbEntries;
 
 // Number of words staged, or -1 when the particle must be re-searched.
