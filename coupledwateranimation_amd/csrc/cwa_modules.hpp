@@ -316,6 +316,10 @@ class CoupledScene : public Module {
     // Neighbour count (self included) of every particle, original order, as
     // the last step's density pass found it.
     void NeighbourCounts(int32_t* out);
+    // The caller wrote particle state through the device pointers: the cell
+    // keys and counts the last step hashed ahead describe old positions, so
+    // the next step hashes afresh.
+    void ParticlesWritten();
     hipStream_t stream() const { return st_; }
     const DevParams& devparams() const { return dp_; }
     // Layout generation: advanced by every Reinit (particle count, heightfield

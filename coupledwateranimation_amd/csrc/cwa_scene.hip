@@ -424,6 +424,11 @@ void CoupledScene::Reinit(const cwa_init& init) {
     hip_check(hipStreamSynchronize(st_), "reinit sync");
 }
 
+void CoupledScene::ParticlesWritten() {
+    grid.ClearCounters(st_);  // stream-ordered after the caller's writes
+    stepped_ = false;         // the neighbour words describe the old positions
+}
+
 void CoupledScene::SetParams(const cwa_params& p) {
     validate_params(p);
     const bool grid_changed = p.grid_dims[0] != params_.grid_dims[0] ||
@@ -535,6 +540,12 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
     // the grid keys and the particle state
     const int cycle = 6;
     int done = 0;
+    // A captured step takes its keys from the step before; after a host-side
+    // change of the particles the first step hashes them eagerly.
+    if (nsteps > 0 && particles.n() > 0 && !grid.keys_ready()) {
+        Step(dt);
+        ++done;
+    }
     // A captured graph replays the buffers of the phase it was captured at:
     // step eagerly until the rotation phase matches it again.
     if (gexec_ && graph_dt_ == dt) {
@@ -983,6 +994,13 @@ int cwa_scene_particles(const cwa_scene* s, cwa_particle_view* v) {
         v->rho = P.rho();
         v->prs = P.prs();
         v->id = P.A.id;
+    });
+}
+
+int cwa_scene_particles_written(cwa_scene* s) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        s->impl->ParticlesWritten();
     });
 }
 

@@ -184,6 +184,11 @@ class Scene:
         L.check(self._lib.cwa_scene_particles(self._h, C.byref(v)))
         return v
 
+    def particles_written(self) -> None:
+        """cwa_scene_particles_written: particle state was written through
+        the device pointers of particles(); the next step hashes afresh."""
+        L.check(self._lib.cwa_scene_particles_written(self._h))
+
     def grid(self) -> L.GridView:
         v = L.GridView()
         L.check(self._lib.cwa_scene_grid(self._h, C.byref(v)))

@@ -137,9 +137,9 @@ typedef struct cwa_init {
 /* Device pointers of the particle buffers (replaces particles_ssbo): the
  * newest state, in the cell order of the last step.  The library alternates
  * two state buffers, so the pointers hold until the next step, reinit or
- * destroy.  Read-only: the last step already hashed these positions into
- * the next step's cell keys, so a position written through the view would
- * be sorted into a stale cell; change the state with cwa_scene_reinit. */
+ * destroy.  The last step already hashed these positions into the next
+ * step's cell keys: after writing positions through the view, call
+ * cwa_scene_particles_written (or change the state with cwa_scene_reinit). */
 typedef struct cwa_particle_view {
     int32_t n;
     float* pos;    /* float4[n] */
@@ -258,6 +258,14 @@ void* cwa_scene_stream(const cwa_scene* s);
 
 /* ---- buffer handles ---- */
 int cwa_scene_particles(const cwa_scene* s, cwa_particle_view* v);
+/* Declares that the caller wrote particle state (positions in particular)
+ * through cwa_scene_particles' pointers (the reference's particle SSBO,
+ * Main.cpp:779-782, is GL storage any GL code may write between dispatches;
+ * no function of the reference corresponds): the next step hashes the positions
+ * afresh instead of using the cell keys the last step computed ahead.  Call
+ * it after the writes are ordered before the scene's next step (on the
+ * scene's stream, or synchronised), and before that step. */
+int cwa_scene_particles_written(cwa_scene* s);
 int cwa_scene_grid(const cwa_scene* s, cwa_grid_view* v);
 int cwa_scene_readback(cwa_scene* s, cwa_host_state* h);
 /* Neighbour count of every particle (original order, n int32 on the host)

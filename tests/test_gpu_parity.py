@@ -708,3 +708,54 @@ def test_changes_between_steps(gpu, cwa, oracle):
     compare("2 steps after the reinit")
     o.close()
     s.close()
+
+
+def test_particles_written_through_the_view(gpu, cwa, oracle):
+    """Positions written into the scene's device buffers between steps, as
+    the reference's host may write its particle SSBO, are hashed afresh
+    after cwa_scene_particles_written: the next step's grid arrays and
+    neighbour counts are bit-exact against the oracle on the written
+    positions, on the eager path and again after a run through the graph
+    path."""
+    import torch
+
+    from coupledwateranimation_amd import configs
+    from coupledwateranimation_amd.scene import _hip
+
+    spec = configs.dam_break_small()
+    p, dt, n = spec.params, spec.dt, spec.n
+    s = spec.make_scene()
+    s.step(dt, 7, graph=True)
+    st = s.readback()
+    ids = s.slot_ids()
+    rng = np.random.default_rng(3)
+    new = st["pos"].copy()
+    new[:, :3] += (rng.uniform(-0.4, 0.4, (n, 3)) * p.h).astype(np.float32)
+    new[:, :3] = np.clip(new[:, :3], np.array(p.lower, np.float32), np.array(p.upper, np.float32))
+    old_keys = oracle.grid_build(p, st["pos"][ids])["key"]
+    slots = np.ascontiguousarray(new[ids])
+    assert np.count_nonzero(oracle.grid_build(p, slots)["key"] != old_keys) > n // 4
+    src = torch.from_numpy(slots).cuda()
+    torch.cuda.synchronize()
+    s.sync()
+    v = s.particles()
+    assert _hip().rt.hipMemcpy(v.pos, src.data_ptr(), slots.nbytes, 3) == 0
+    s.particles_written()
+
+    def check(pos_orig, pos_slots, label):
+        g = s.grid_host()
+        r = oracle.grid_build(p, pos_slots)
+        for k in ("key", "count", "offset", "list"):
+            assert np.array_equal(g[k], r[k]), (label, k)
+        nb = s.neighbour_counts()
+        assert np.array_equal(nb, oracle.neighbor_counts(p, pos_orig, all_pairs=False)), label
+        _say(f"{label}: grid and neighbour counts bit-exact")
+
+    s.step(dt, 1)
+    check(new, slots, "first step after the write")
+    s.step(dt, 7, graph=True)
+    st = s.readback()
+    slots = st["pos"][s.slot_ids()]
+    s.step(dt, 1)
+    check(st["pos"], slots, "after 7 more steps")
+    s.close()
