@@ -364,7 +364,12 @@ def search_stats(p, pos, nb) -> dict:
     x, y, z = (pos[:, a].astype(f32) for a in range(3))
     ix, iy, iz = axis(x, gmin[0], cell, dims[0]), axis(y, gmin[1], cy, dims[1]), \
         axis(z, gmin[2], cell, dims[2])
-    key = (ix * dims[2] + iz) * dims[1] + iy
+    def column(cx, cz):  # the device's z-tiled column order (16-column tiles)
+        t = cz // 16
+        w = np.minimum(16, dims[2] - t * 16)
+        return t * 16 * dims[0] + cx * w + (cz - t * 16)
+
+    key = column(ix, iz) * dims[1] + iy
     order = np.argsort(key, kind="stable")
     offset = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=dims[0] * dims[1] *
                                                         dims[2]))])
@@ -386,7 +391,7 @@ def search_stats(p, pos, nb) -> dict:
             ry = np.sqrt(np.maximum(T - d2, f32(0)))
             ylo, yhi = axis(y - ry, gmin[1], cy, dims[1]), axis(y + ry, gmin[1], cy, dims[1])
             cxc, czc = np.clip(cx, 0, dims[0] - 1), np.clip(cz, 0, dims[2] - 1)
-            base = (cxc * dims[2] + czc) * dims[1]
+            base = column(cxc, czc) * dims[1]
             ln = np.where(ok, offset[base + yhi + 1] - offset[base + ylo], 0)
             total += ln
             wave_cost += int(np.concatenate([ln, np.zeros(pad, np.int64)]).reshape(-1, 64)

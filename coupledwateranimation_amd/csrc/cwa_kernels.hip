@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(256) k_box_pairs(const float4* __restrict__ x,
     for (int a = lo[0]; a <= hi[0]; ++a)
         for (int b = lo[1]; b <= hi[1]; ++b)
             for (int c = lo[2]; c <= hi[2]; ++c) {
-                pkey[p] = lin_cell(g.dims, a, b, c);
+                pkey[p] = (a * g.dims[2] + c) * g.dims[1] + b;  // untiled, y fastest
                 pbox[p] = i;
                 ++p;
             }
@@ -650,9 +650,10 @@ __device__ __forceinline__ void force_pair(float h, float4 pi, float4 vi, float 
 }
 
 // ---------------------------------------------------------------------------
-// Particle -> wave flux per vertical column of grid cells.  Column (ix, iz)
-// is one contiguous sorted range; one wave64 sums it (lane-strided partial
-// sums, then a fixed xor tree: deterministic).  Output at [iz*nx + ix].
+// Particle -> wave flux per vertical column of grid cells.  Each column is
+// one contiguous sorted range; one wave64 sums memory column `wave`
+// (lane-strided partial sums, then a fixed xor tree: deterministic) and
+// stores it at [iz*nx + ix] of its (ix, iz).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void column_flux_sum(const int* __restrict__ offset,
                                                 const float* __restrict__ flux, int nx, int ny,
@@ -665,7 +666,9 @@ __device__ __forceinline__ void column_flux_sum(const int* __restrict__ offset,
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) {
-        const int ix = wave / nz, iz = wave - ix * nz;
+        const int dims[3] = {nx, ny, nz};
+        int ix, iz;
+        grid_column_xz(dims, wave, ix, iz);
         colflux[iz * nx + ix] = s;
     }
 }

@@ -150,7 +150,8 @@ typedef struct cwa_particle_view {
     int32_t* id;   /* int[n], original particle index per slot */
 } cwa_particle_view;
 
-/* Device pointers of the uniform grid of the last step. */
+/* Device pointers of the uniform grid of the last step (cell order as
+ * cwa_grid_build's). */
 typedef struct cwa_grid_view {
     int32_t dims[3];
     int32_t num_cells;
@@ -284,7 +285,11 @@ int cwa_scan_exclusive_i32(const int32_t* d_in, int32_t* d_out, int n, void* str
 /* UgridParticles3D::BuildGrid (UniformGridParticles3D.cpp:170-211) with the
  * stable insertion order of UniformGrid2D::Build (UniformGrid2D.cpp:33-92):
  * key[i] = linear cell of pos[i], count, offset (num_cells+1), list = indices
- * sorted by (cell, index).  Linear cell = (ix*dims_z + iz)*dims_y + iy. */
+ * sorted by (cell, index).  Linear cell = column(ix, iz)*dims_y + iy, y
+ * fastest; columns in z-tiles of 16, x-major within a tile (the last tile
+ * narrower when 16 does not divide dims_z): column(ix, iz) =
+ * t*16*dims_x + ix*w + (iz - 16 t), t = iz/16, w = min(16, dims_z - 16 t).
+ * The scene's grid (cwa_scene_grid) uses the same order. */
 int cwa_grid_build(const float* d_pos4, int n, const float grid_min[3], float cell_size,
                    const int32_t dims[3], int32_t* d_key, int32_t* d_count,
                    int32_t* d_offset, int32_t* d_list, void* stream);

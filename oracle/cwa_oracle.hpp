@@ -194,9 +194,12 @@ static inline void world_uv(const Consts<R>& c, R x, R z, R& u, R& v) {
 
 // ---------------------------------------------------------------------------
 // Uniform grid (ugrid_particles_cs.glsl:97-108; UniformGrid2D.cpp:25-92).
-// Linearisation is y fastest: lin = (ix*nz + iz)*ny + iy, so every vertical
-// column of cells is one contiguous range (the reference's Index() uses
-// (i*ny + j)*nx + k, which aliases cells on non-cubic grids).
+// Linearisation is y fastest, so every vertical column of cells is one
+// contiguous range (the reference's Index() uses (i*ny + j)*nx + k, which
+// aliases cells on non-cubic grids); the columns in z-tiles of kTileZ,
+// x-major within a tile, the last tile possibly narrower:
+//   column(ix, iz) = t*kTileZ*nx + ix*w + (iz - t*kTileZ),
+//   t = iz / kTileZ, w = min(kTileZ, nz - t*kTileZ);  lin = column*ny + iy.
 // ---------------------------------------------------------------------------
 template <class R>
 static inline void cell_of(const Consts<R>& c, R px, R py, R pz, int& ix, int& iy, int& iz) {
@@ -214,7 +217,19 @@ static inline void cell_of(const Consts<R>& c, R px, R py, R pz, int& ix, int& i
     iz = ci[2];
 }
 
+constexpr int kTileZ = 16;
+static inline int grid_column(const int dims[3], int ix, int iz) {
+    const int t = iz / kTileZ;
+    const int w = std::min(kTileZ, dims[2] - t * kTileZ);
+    return t * kTileZ * dims[0] + ix * w + (iz - t * kTileZ);
+}
+
 static inline int lin_cell(const int dims[3], int ix, int iy, int iz) {
+    return grid_column(dims, ix, iz) * dims[1] + iy;
+}
+
+// The box grid keeps the plain order (ix*nz + iz)*ny + iy.
+static inline int box_lin_cell(const int dims[3], int ix, int iy, int iz) {
     return (ix * dims[2] + iz) * dims[1] + iy;
 }
 
@@ -296,7 +311,8 @@ static void grid_build(const Consts<R>& c, const R* pos4, int n, int32_t* key, i
 // cell size per axis), the exclusive scan of the counts (ParallelScan), and
 // the insertion made stable: boxes ascending within a cell, where the
 // shader's atomicAdd leaves the order open.  The cell is linearised y
-// fastest like the particle grid (Index() at :175-178 uses (i*ny + j)*nx + k).
+// fastest, (ix*nz + iz)*ny + iy, without the particle grid's z-tiles
+// (Index() at :175-178 uses (i*ny + j)*nx + k).
 // Arithmetic in float, as the shader: x - hw and x + hw rounded, then the
 // correctly rounded division.  Returns the number of (box, cell) pairs;
 // list is written only when it holds that many (capacity).
@@ -325,7 +341,8 @@ static long long grid_build_boxes(const float* x4, const float* hw4, int n, cons
         total += m;
         for (int a = lo[3 * i]; a <= hi[3 * i]; ++a)
             for (int b = lo[3 * i + 1]; b <= hi[3 * i + 1]; ++b)
-                for (int c = lo[3 * i + 2]; c <= hi[3 * i + 2]; ++c) count[lin_cell(dims, a, b, c)]++;
+                for (int c = lo[3 * i + 2]; c <= hi[3 * i + 2]; ++c)
+                    count[box_lin_cell(dims, a, b, c)]++;
     }
     exclusive_scan(count, nc, offset);
     if (total > capacity) return total;
@@ -334,7 +351,7 @@ static long long grid_build_boxes(const float* x4, const float* hw4, int n, cons
         for (int a = lo[3 * i]; a <= hi[3 * i]; ++a)
             for (int b = lo[3 * i + 1]; b <= hi[3 * i + 1]; ++b)
                 for (int c = lo[3 * i + 2]; c <= hi[3 * i + 2]; ++c) {
-                    const int k = lin_cell(dims, a, b, c);
+                    const int k = box_lin_cell(dims, a, b, c);
                     list[offset[k] + fill[k]++] = i;
                 }
     return total;
@@ -766,8 +783,8 @@ static void scene_step(Scene<R>& S, R dt) {
     }
 
     // Column flux: column (ix, iz) is the contiguous cell range
-    // [(ix*nz + iz)*ny, (ix*nz + iz + 1)*ny) in the y-fastest order, so its
-    // particles are one contiguous sorted range, summed in slot order.
+    // [col*ny, (col + 1)*ny), col = grid_column(ix, iz), so its particles are
+    // one contiguous sorted range, summed in slot order.
     // Stored at [iz*nx + ix] (x fastest, like the heightfield).
     const bool couple = c.band > R(0) && S.mode == 0;
     const int nxc = c.dims[0], nzc = c.dims[2];
@@ -775,7 +792,7 @@ static void scene_step(Scene<R>& S, R dt) {
     if (couple) {
         for (int ix = 0; ix < nxc; ++ix) {
             for (int iz = 0; iz < nzc; ++iz) {
-                const int col = ix * nzc + iz;
+                const int col = grid_column(c.dims, ix, iz);
                 const int b = S.offset[col * c.dims[1]];
                 const int e = S.offset[(col + 1) * c.dims[1]];
                 R sum = 0;

@@ -65,13 +65,63 @@ def _check_grid_invariants(g, n, nc, key_ref):
         assert offset[c + 1] - offset[c] == np.count_nonzero(key == c)
 
 
+TILE_Z = 16
+
+
+def _column(ix, iz, nx, nz):
+    """Memory column of cell column (ix, iz): z-tiles of 16 columns, x-major
+    within a tile, the last tile narrower when 16 does not divide nz."""
+    t = iz // TILE_Z
+    w = np.minimum(TILE_Z, nz - t * TILE_Z)
+    return t * TILE_Z * nx + ix * w + (iz - t * TILE_Z)
+
+
 def _hash_ref(pos, p):
-    """Closed-form cell hash, y fastest: ((ix*nz + iz)*ny + iy)."""
+    """Closed-form cell hash, y fastest: column(ix, iz)*ny + iy."""
     d = np.array(p.grid_dims)
     cell = np.float32(p.cell_size)
     q = np.floor((pos[:, :3] - np.array(p.grid_min, np.float32)) / cell)
     q = np.clip(q, 0, d - 1).astype(np.int64)
-    return ((q[:, 0] * d[2] + q[:, 2]) * d[1] + q[:, 1]).astype(np.int32)
+    return (_column(q[:, 0], q[:, 2], d[0], d[2]) * d[1] + q[:, 1]).astype(np.int32)
+
+
+@pytest.mark.parametrize("nx,nz", [(1, 1), (3, 2), (5, 16), (4, 17), (7, 37), (2, 48)])
+def test_column_order_is_a_tiled_bijection(nx, nz):
+    """The column order covers 0..nx*nz-1 once; within a tile the x-major
+    order holds (consecutive columns step z inside the tile), and tiles
+    follow each other: every column of tile t comes before tile t+1."""
+    ix, iz = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    col = _column(ix, iz, nx, nz)
+    assert sorted(col.ravel().tolist()) == list(range(nx * nz))
+    t = iz // TILE_Z
+    for a in range(nx):
+        for b in range(nz - 1):
+            if t[a, b] == t[a, b + 1]:
+                assert col[a, b + 1] == col[a, b] + 1
+    for tt in range(t.max()):
+        assert col[t == tt].max() < col[t == tt + 1].min()
+
+
+def test_cell_hash_tiled_columns(oracle, cwa):
+    """Keys of a grid with 37 columns along z (tiles of 16, 16 and 5) and
+    14 along x, spelled out at tile edges and checked on random points."""
+    p = _params(cwa, dims=(14, 3, 37), cell=0.1, gmin=(0.0, 0.0, 0.0))
+    pts = np.array([[0.05, 0.05, 0.05],      # (0, 0, 0)
+                    [0.05, 0.05, 1.55],      # (0, 0, 15): last column of tile 0
+                    [0.05, 0.05, 1.65],      # (0, 0, 16): first column of tile 1
+                    [0.15, 0.25, 1.65],      # (1, 2, 16)
+                    [1.35, 0.05, 3.65]],     # (13, 0, 36): the last column
+                   np.float32)
+    pos = np.concatenate([pts, np.ones((len(pts), 1), np.float32)], 1)
+    g = oracle.grid_build(p, pos)
+    ny = 3
+    assert g["key"].tolist() == [0, 15 * ny, (16 * 14) * ny, (16 * 14 + 16 + 0) * ny + 2,
+                                 (2 * 16 * 14 + 13 * 5 + 4) * ny]
+    rng = np.random.default_rng(5)
+    pos = np.zeros((3000, 4), np.float32)
+    pos[:, :3] = rng.uniform(-0.1, 3.8, (3000, 3)).astype(np.float32)
+    g = oracle.grid_build(p, pos)
+    _check_grid_invariants(g, 3000, 14 * 3 * 37, _hash_ref(pos, p))
 
 
 @pytest.mark.parametrize("n", [0, 1, 5, 257, 5000])
@@ -98,7 +148,8 @@ def test_cell_hash_edge_cases(oracle, cwa):
     pos = np.concatenate([pts, np.ones((len(pts), 1), np.float32)], 1)
     g = oracle.grid_build(p, pos)
     assert g["key"].tolist() == _hash_ref(pos, p).tolist()
-    # spelled out: (ix, iy, iz) -> (ix*nz + iz)*ny + iy with nz=2, ny=3
+    # spelled out: (ix, iy, iz) -> (ix*nz + iz)*ny + iy with nz=2, ny=3 (one
+    # z-tile narrower than 16 columns)
     lin = lambda ix, iy, iz: (ix * 2 + iz) * 3 + iy  # noqa: E731
     assert g["key"].tolist() == [lin(0, 0, 0), lin(1, 1, 1), lin(0, 0, 0), lin(3, 2, 1),
                                  lin(3, 2, 1), lin(1, 0, 1)]
@@ -123,6 +174,21 @@ def test_neighbor_sets_grid_equals_all_pairs(oracle, cwa, seed, ydiv):
     b = oracle.neighbor_counts(p, pos, all_pairs=True)
     assert a.tolist() == b.tolist()
     assert a.min() >= 1  # self included
+
+
+def test_neighbor_sets_across_z_tiles(oracle, cwa):
+    """The grid search stays exact where neighbour columns lie in another
+    z-tile of the column order: 37 columns along z (tiles of 16, 16, 5)."""
+    rng = np.random.default_rng(9)
+    n = 4000
+    h = 0.03
+    p = _params(cwa, dims=(6, 8, 37), cell=0.032, h=h)
+    pos = np.ones((n, 4), np.float32)
+    pos[:, :3] = rng.uniform([0.0, 0.0, 0.0], [0.19, 0.25, 1.18], (n, 3)).astype(np.float32)
+    a = oracle.neighbor_counts(p, pos, all_pairs=False)
+    b = oracle.neighbor_counts(p, pos, all_pairs=True)
+    assert a.tolist() == b.tolist()
+    assert a.mean() > 3
 
 
 # ---------------------------------------------------------------- box grid
