@@ -62,11 +62,13 @@ namespace b2d {
 constexpr int kNumQueues = 8;   // XCDs of an MI355X
 // Candidates loaded ahead in the density / force loops (the lanes of a tile
 // walk the same candidates, so each load is one address per wave).  Measured
-// on the 256-scene workload (MI355X): density 1, 2, 4, 6, 8, 16 ->
+// on the 256-scene workload (MI355X), cells of H: density 1, 2, 4, 6, 8, 16 ->
 // 1.42, 1.25, 1.16, 1.13, 1.12, 1.13 ms; force 1, 2, 3, 4, 6, 8 ->
-// 2.63, 2.58, 2.51, 2.49, 2.45, 2.45 ms.
+// 2.63, 2.58, 2.51, 2.49, 2.45, 2.45 ms.  With the shorter columns of H/2
+// cells: density 4 / 8 -> 0.921 / 0.926 ms; force 4, 5, 6, 8 -> 1.885,
+// 1.884, 1.898, 1.980 ms.
 constexpr int kDensityBatch2D = 8;
-constexpr int kForceBatch2D = 6;
+constexpr int kForceBatch2D = 4;
 constexpr int kRunShift = 26;   // run record: slot | (len - 1) << 26
 constexpr unsigned kRunSlotMask = (1u << kRunShift) - 1u;
 
