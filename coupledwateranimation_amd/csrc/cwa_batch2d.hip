@@ -67,7 +67,7 @@ constexpr int kNumQueues = 8;   // XCDs of an MI355X
 // 2.63, 2.58, 2.51, 2.49, 2.45, 2.45 ms.  With the shorter columns of H/2
 // cells: density 4 / 8 -> 0.921 / 0.926 ms; force 4, 5, 6, 8 -> 1.885,
 // 1.884, 1.898, 1.980 ms.
-constexpr int kDensityBatch2D = 8;
+constexpr int kDensityBatch2D = 4;
 constexpr int kForceBatch2D = 4;
 constexpr int kRunShift = 26;   // run record: slot | (len - 1) << 26
 constexpr unsigned kRunSlotMask = (1u << kRunShift) - 1u;
