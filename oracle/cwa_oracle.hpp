@@ -7,8 +7,7 @@
 // Templated on the real type R (float: the reference's arithmetic; double:
 // the FP64 twin used to report each field's FP32 error).  Every function
 // cites the reference file:line it restates.  Where the reference is
-// ambiguous or quirky the choice is named in the comment (SURVEY appendix A
-// items are restated here as facts).
+// ambiguous or quirky the choice is stated in the comment beside it.
 //
 // Parity status (what pins each part; the tests name the checks):
 //   texture sampler ........ pinned: affine fields reproduced, clamp at edges
@@ -19,7 +18,8 @@
 //   exclusive scan ......... pinned: reference KAT (ParallelScan.cpp:129-137)
 //                            and the reference's Blelloch tree restated
 //   grid build ............. pinned: permutation / stability invariants,
-//                            brute-force neighbour sets
+//                            the z-tiled column order spelled out,
+//                            brute-force neighbour sets (also across tiles)
 //   box grid build ......... pinned: brute-force interval overlap of every
 //                            box with every cell, clamping at the edges
 //   density / EOS .......... pinned: isolated particle, two-particle closed
@@ -27,7 +27,7 @@
 //   forces ................. pinned: two-particle closed form, isolated
 //                            particle (gravity + torque + drag + wave)
 //   integrate / boundary ... pinned: closed forms per face, foam branch,
-//                            surface clamp
+//                            surface clamp, the wake-mode ceiling's cases
 //   particle->wave flux .... pinned: single-particle closed form (this term
 //                            has no reference counterpart)
 //   full trajectories ...... parity unpinned against the reference itself
