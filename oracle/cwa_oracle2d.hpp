@@ -38,9 +38,22 @@
 //   * Pair terms skip coincident particles (r = 0), where rij/r is 0/0.
 //   * The grid's cells are H/2 wide: the query visits the 5 cell columns
 //     ix-2..ix+2 around the particle's column (column order, slots ascending
-//     within a column, rows iy-2..iy+2), the same neighbours within H as the
-//     reference's (pos -+ H) cell box on its 0.3-wide cells, in another
-//     order.  The particle -> wave flux is summed per column pair (H wide).
+//     within a column), each cut to the rows its disc of radius H reaches
+//     (column_rows: float bounds with a correctly rounded square root and a
+//     margin, the same operations as the kernels), the same neighbours
+//     within H as the reference's (pos -+ H) cell box on its 0.3-wide
+//     cells, in another order.  The particle -> wave flux is summed per
+//     column pair (H wide).
+// Pinned (tests/test_oracle2d.py): kernel normalisation by quadrature and its
+// gradient by differences; viscosity as nu * Laplacian of a quadratic
+// velocity field on a lattice, radial and dissipative per pair; the psi
+// boundary terms at known wall distances and the boundary pressure as the
+// difference gradient of the boundary density; the wave pressure's gate and
+// smoothstep edges; SDF distances and ties; the wave's closed forms
+// (uniform field, CFL-1 shift, free boundary, attenuation); Shallow1D at
+// rest, mass telescoping and pulse speed; the stable sort; flux
+// conservation.  Full trajectories against the reference: parity unpinned
+// (the GLSL cannot run here).
 #pragma once
 
 #include <algorithm>
