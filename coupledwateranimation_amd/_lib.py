@@ -171,6 +171,7 @@ SIGNATURES = {
     "cwa_last_error": (C.c_char_p, []),
     "cwa_kernel_name": (C.c_char_p, [C.c_int]),
     "cwa_scene_particles_written": (C.c_int, [C.c_void_p]),
+    "cwa_scene_prepare_graph": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_int)]),
     "cwa_set_lane_split_limit": (C.c_int, [C.c_longlong, C.POINTER(C.c_longlong)]),
     "cwa_params_reference": (None, [C.POINTER(Params)]),
     "cwa_scene_create": (C.c_int, [C.POINTER(Params), C.POINTER(Init), C.POINTER(C.c_void_p)]),

@@ -308,6 +308,13 @@ class CoupledScene : public Module {
     // tape: optional extra outputs for the adjoint.
     void Step(float dt, hipEvent_t (*events)[2] = nullptr, const TapeOut* tape = nullptr);
     void StepGraph(float dt, int nsteps);
+    // Capture StepGraph's replay graph for dt without running a step; false
+    // (nothing captured) before the first step or with check_finite set.
+    bool PrepareGraph(float dt);
+    // Period of the host-side buffer rotations: the wave triple buffer (3) x
+    // the double buffers (2) of the grid counters, the grid keys and the
+    // particle state.
+    static constexpr int kGraphCycle = 6;
     void WaveOnly(float dt);
     void WaveReinit();
     void WaveReinitFromTexture(const float* tex, int tw, int th);
@@ -361,6 +368,7 @@ class CoupledScene : public Module {
    private:
     void build_devparams();
     void invalidate_graph();
+    void CaptureGraph(float dt);
     cwa_params params_{};
     DevParams dp_{};
     DeviceBuffer<DevParams> dpd_;  // device mirror read by the kernels

@@ -531,14 +531,45 @@ void CoupledScene::CheckFinite() {
     if (h) throw Error(CWA_ENONFINITE, "non-finite particle or height value after step");
 }
 
+// Records kGraphCycle steps into gexec_ without running them.  The host-side
+// rotations have period kGraphCycle, so the capture leaves them where it
+// found them, and the graph replays from this phase.
+void CoupledScene::CaptureGraph(float dt) {
+    invalidate_graph();
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
+    try {
+        for (int k = 0; k < kGraphCycle; ++k) Step(dt);
+    } catch (...) {
+        (void)hipStreamEndCapture(st_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(st_, &g), "end capture");
+    const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(e, "graph instantiate");
+    graph_dt_ = dt;
+    graph_phase_ = phase();
+}
+
+bool CoupledScene::PrepareGraph(float dt) {
+    if (params_.check_finite) return false;  // StepGraph runs eagerly then
+    // the captured steps take their keys from the step before them
+    if (particles.n() > 0 && !grid.keys_ready()) return false;
+    if (gexec_ && graph_dt_ == dt && graph_phase_ == phase()) return true;
+    const bool stepped = stepped_;  // nothing runs: the last step's records stay current
+    CaptureGraph(dt);
+    stepped_ = stepped;
+    return true;
+}
+
 void CoupledScene::StepGraph(float dt, int nsteps) {
     if (params_.check_finite) {  // the finite check synchronises: run eagerly
         for (int k = 0; k < nsteps; ++k) Step(dt);
         return;
     }
-    // wave triple buffer (3) x the double buffers (2) of the grid counters,
-    // the grid keys and the particle state
-    const int cycle = 6;
+    const int cycle = kGraphCycle;
     int done = 0;
     // A captured step takes its keys from the step before; after a host-side
     // change of the particles the first step hashes them eagerly.
@@ -563,24 +594,8 @@ void CoupledScene::StepGraph(float dt, int nsteps) {
         ++done;
     }
     if (nsteps - done >= cycle && (!gexec_ || graph_dt_ != dt) &&
-        (particles.n() == 0 || grid.keys_ready())) {
-        invalidate_graph();
-        hipGraph_t g = nullptr;
-        hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture");
-        try {
-            for (int k = 0; k < cycle; ++k) Step(dt);
-        } catch (...) {
-            (void)hipStreamEndCapture(st_, &g);
-            if (g) (void)hipGraphDestroy(g);
-            throw;
-        }
-        hip_check(hipStreamEndCapture(st_, &g), "end capture");
-        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        hip_check(e, "graph instantiate");
-        graph_dt_ = dt;
-        graph_phase_ = phase();  // capture advanced the host state a full cycle
-    }
+        (particles.n() == 0 || grid.keys_ready()))
+        CaptureGraph(dt);
     while (gexec_ && graph_dt_ == dt && phase() == graph_phase_ && nsteps - done >= cycle) {
         hip_check(hipGraphLaunch(gexec_, st_), "graph launch");
         done += cycle;
@@ -936,6 +951,15 @@ int cwa_scene_step_graph(cwa_scene* s, float dt, int nsteps) {
         CWA_REQUIRE(nsteps >= 0, "nsteps < 0");
         CWA_REQUIRE(std::isfinite(dt), "dt must be finite");
         s->impl->StepGraph(dt, nsteps);
+    });
+}
+
+int cwa_scene_prepare_graph(cwa_scene* s, float dt, int* captured) {
+    return guarded([&] {
+        CWA_REQUIRE(s, "null argument");
+        CWA_REQUIRE(std::isfinite(dt), "dt must be finite");
+        const bool c = s->impl->PrepareGraph(dt);
+        if (captured) *captured = c ? 1 : 0;
     });
 }
 

@@ -184,6 +184,13 @@ class Scene:
         L.check(self._lib.cwa_scene_particles(self._h, C.byref(v)))
         return v
 
+    def prepare_graph(self, dt: float) -> bool:
+        """cwa_scene_prepare_graph: capture the step graph for dt without
+        running a step; False when none can be captured yet."""
+        c = C.c_int()
+        L.check(self._lib.cwa_scene_prepare_graph(self._h, float(dt), C.byref(c)))
+        return bool(c.value)
+
     def particles_written(self) -> None:
         """cwa_scene_particles_written: particle state was written through
         the device pointers of particles(); the next step hashes afresh."""

@@ -253,6 +253,14 @@ int cwa_scene_step_profiled(cwa_scene* s, float dt, int nsteps, cwa_kernel_times
  * launch it; nsteps that are not a multiple of 6 run the tail eagerly.
  * The graph is re-captured when dt changes. */
 int cwa_scene_step_graph(cwa_scene* s, float dt, int nsteps);
+/* Captures cwa_scene_step_graph's 6-step graph for dt now, without running
+ * a step, so later cwa_scene_step_graph calls with this dt only replay it
+ * (for loops that must not pay the capture inside a timed or latency-bound
+ * region; no reference counterpart).  *captured (optional) = 1 when a graph
+ * is ready, 0 when none could be captured yet: before the scene's first step
+ * (a captured step takes its cell keys from the step before it) or with
+ * check_finite set (those steps run eagerly). */
+int cwa_scene_prepare_graph(cwa_scene* s, float dt, int* captured);
 int cwa_scene_sync(cwa_scene* s);
 /* The scene's hipStream_t (as void*). */
 void* cwa_scene_stream(const cwa_scene* s);

@@ -524,6 +524,33 @@ def test_graph_equals_eager_and_deterministic(gpu, cwa):
         assert np.array_equal(a[k], c[k]), k
 
 
+def test_prepared_graph_equals_eager(gpu, cwa):
+    """cwa_scene_prepare_graph captures the step graph without running a
+    step (the state is unchanged by it); stepping through the prepared graph
+    then ends bit-identical to eager stepping.  Before the first step
+    nothing is captured (a captured step takes its keys from the step
+    before)."""
+    spec = _specs(cwa)["dam_break_4k"]
+    dt = spec.dt
+    a, b = spec.make_scene(), spec.make_scene()
+    assert not a.prepare_graph(dt)
+    a.step(dt, 5, graph=True)
+    b.step(dt, 5)
+    st0 = a.readback()
+    assert a.prepare_graph(dt)
+    assert a.prepare_graph(dt)  # already prepared at this phase
+    st1 = a.readback()
+    for k in st0:
+        assert np.array_equal(st0[k], st1[k]), k
+    a.step(dt, 20, graph=True)
+    b.step(dt, 20)
+    sa, sb = a.readback(), b.readback()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
 def test_graph_after_eager_tail(gpu, cwa):
     """A graph captured at one phase of the buffer rotations is replayed
     only at that phase: graph calls with ragged step counts (eager tails in
