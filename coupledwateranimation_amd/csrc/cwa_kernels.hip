@@ -711,12 +711,12 @@ __global__ void __launch_bounds__(256) k_force_integrate(
     if (valid) {
         const float4 own_a = nrec[2 * (size_t)i];      // (x, y, z, p)
         const float4 own_b = nrec[2 * (size_t)i + 1];  // (vx, vy, vz, 1/rho)
-        const int src = list[i];                       // the particle's slot in the state
+        const int src = load_once(&list[i]);           // the particle's slot in the state
         // the previous force is used only after the neighbour loop; issued
         // here, its load is in flight while the loop runs
-        float4 fp = force[src];
-        const float4 pi = make_float4(own_a.x, own_a.y, own_a.z, bpos[i].w);
-        const float4 vi = make_float4(own_b.x, own_b.y, own_b.z, vpad[i]);
+        float4 fp = load_once(&force[src]);
+        const float4 pi = make_float4(own_a.x, own_a.y, own_a.z, load_once(&bpos[i].w));
+        const float4 vi = make_float4(own_b.x, own_b.y, own_b.z, load_once(&vpad[i]));
         const float p_i = own_a.w;
         const float h = P.h;
         const float m_spiky_half = P.m * P.spiky * 0.5f;
@@ -794,7 +794,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
             wfy = -height * ny * P.wforce;
             wfz = -height * -1.0f * P.wforce;
         }
-        float rho = rho_w[i];
+        float rho = load_once(&rho_w[i]);
         float prs = p_i;
         float Fx = a.pfx + vfx + rho * P.g[0] + tqx + drx + wfx;
         float Fy = a.pfy + vfy + rho * P.g[1] + tqy + dry + wfy;
@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(256) k_force_integrate(
         pos_o[i] = make_float4(npx, npy, npz, pi.w);
         vel_o[i] = make_float4(nvx, nvy, nvz, vi.w);
         force_o[i] = make_float4(Fx, Fy, Fz, Fw);
-        id_o[i] = bid[i];
+        id_o[i] = load_once(&bid[i]);
         rho_o[i] = rho;
         prs_o[i] = prs;
         if (next.key) {
