@@ -355,8 +355,11 @@ __global__ void __launch_bounds__(256) k_stable_reorder(
     if (!valid || d < 0 || d >= n) return;  // out of range only on inconsistent run records
     list[d] = item_of ? item_of[s] : s;
     if (pos_out) {
-        pos_out[d] = pos_in[s];
-        id_out[d] = id_in[s];
+        // read once (nt): the previous force pass's output leaves the L2 to the
+        // sorted copy this pass writes and the next passes gather from (A/B
+        // on one box: step 0.2606 -> 0.2548 ms, the force pass 109.6 -> 104.6 us)
+        pos_out[d] = load_once(&pos_in[s]);
+        id_out[d] = load_once(&id_in[s]);
     }
 }
 
