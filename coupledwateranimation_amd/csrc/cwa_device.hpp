@@ -117,6 +117,10 @@ __device__ __forceinline__ uint2 load_once(const uint2* p) {
     return make_uint2(v.x, v.y);
 }
 
+// Stores of data no pass of the next step reads (the visible density and
+// pressure): nt, so they do not displace what it does read.
+__device__ __forceinline__ void store_once(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ int lin_cell(const int dims[3], int ix, int iy, int iz) {
     return grid_column(dims, ix, iz) * dims[1] + iy;
 }

@@ -840,8 +840,8 @@ __global__ void __launch_bounds__(256) k_force_integrate(
         vel_o[i] = make_float4(nvx, nvy, nvz, vi.w);
         force_o[i] = make_float4(Fx, Fy, Fz, Fw);
         id_o[i] = load_once(&bid[i]);
-        rho_o[i] = rho;
-        prs_o[i] = prs;
+        store_once(&rho_o[i], rho);  // not read by the next step (A/B: step -0.7 %)
+        store_once(&prs_o[i], prs);
         if (next.key) {
             const int ix = cell_axis(npx, P.gmin[0], P.cell, P.dims[0]);
             const int iy = cell_axis(npy, P.gmin[1], P.cell_y, P.dims[1]);
