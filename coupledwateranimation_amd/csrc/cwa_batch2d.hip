@@ -174,6 +174,23 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// 1 / b correctly rounded (bit-identical to IEEE division) for
+// 2^-40 <= b <= 2^40: the hardware reciprocal and two Newton-Raphson steps
+// y + y (1 - b y), each with two fused multiply-adds.  From a start within
+// 3 ulp these give the correctly rounded reciprocal for every b in that
+// range but one kind, a mantissa of all ones, b = (2 - 2^-23) 2^e: there
+// 1 / b lies just above the midpoint 2^-(e+1) (1 + 2^-24), the step lands
+// exactly on it and rounds to the even 2^-(e+1), whose last mantissa bit
+// is the one missing.  The carry out of mantissa + 1 is that bit.  Checked
+// exhaustively over all 81 binades for starts up to 3 ulp either side.
+__device__ __forceinline__ float recip_rn(float b) {
+    float y = __builtin_amdgcn_rcpf(b);
+    y = fmaf(fmaf(-b, y, 1.0f), y, y);
+    y = fmaf(fmaf(-b, y, 1.0f), y, y);
+    const unsigned ub = __float_as_uint(b);
+    return __uint_as_float(__float_as_uint(y) | (((ub & 0x7FFFFFu) + 1u) >> 23));
+}
+
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
     const float f = clampf_nan(floorf((p - lo) / H), -1.0f, (float)dim);
     return min(max((int)f, 0), dim - 1);
@@ -461,14 +478,15 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
         for (int cx = max(ix - 2, 0); cx <= min(ix + 2, P.nx - 1); ++cx) {
             int b, e;
             column_rows(P, A.off, cb, x, y, ix, cx, b, e);
-            auto pair = [&](int j, const float4 qj) {
+            auto pair = [&](const float4 qj, const float2 bj) {
                 const float dx = x - qj.x, dy = y - qj.y;
                 const float r2 = fmaf(dy, dy, dx * dx);
-                if (j != i && r2 < P.T && r2 > 0.0f) {
-                    const float2 bj = A.aj[j];
+                if (r2 < P.T && r2 > 0.0f) {  // r2 > 0 also excludes j == i
                     const float r = sqrt_rn(r2);
                     const float wg = W_cubic_grad(P, r);
-                    const float inv = 1.0f / r;
+                    // outside [2^-40, 2^40] the division itself
+                    const bool wide = r2 < 0x1p-80f || r2 > 0x1p80f;
+                    const float inv = __builtin_expect(wide, 0) ? 1.0f / r : recip_rn(r);
                     const float ux = dx * inv, uy = dy * inv;
                     const float s = (ai + bj.x) * wg;
                     apx -= s * ux;
@@ -479,17 +497,23 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                     avy -= tv * uy;
                 }
             };
-            // kForceBatch2D candidates loaded before the first is tested;
-            // tested and accumulated in order
+            // kForceBatch2D candidates (position / velocity and the pressure
+            // and 1/rho pair) loaded before the first is tested; tested and
+            // accumulated in order.  Loading the pair of every candidate, not
+            // only of the neighbours, trades a few cached loads for the
+            // latency each neighbour otherwise waits on.
             int j = b;
             for (; j + kForceBatch2D <= e; j += kForceBatch2D) {
                 float4 qj[kForceBatch2D];
 #pragma unroll
                 for (int u = 0; u < kForceBatch2D; ++u) qj[u] = A.q[j + u];
+                float2 bj[kForceBatch2D];
 #pragma unroll
-                for (int u = 0; u < kForceBatch2D; ++u) pair(j + u, qj[u]);
+                for (int u = 0; u < kForceBatch2D; ++u) bj[u] = A.aj[j + u];
+#pragma unroll
+                for (int u = 0; u < kForceBatch2D; ++u) pair(qj[u], bj[u]);
             }
-            for (; j < e; ++j) pair(j, A.q[j]);
+            for (; j < e; ++j) pair(A.q[j], A.aj[j]);
         }
         const float cv = P.visc_pre * S.mass;
         avx *= cv;
