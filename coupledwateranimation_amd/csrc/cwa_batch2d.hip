@@ -191,6 +191,31 @@ __device__ __forceinline__ float recip_rn(float b) {
     return __uint_as_float(__float_as_uint(y) | (((ub & 0x7FFFFFu) + 1u) >> 23));
 }
 
+// sqrt(x) correctly rounded for 2^-96 <= x <= 2^96, without compares:
+// the hardware reciprocal square root y, g = x y and h = y / 2, one coupled
+// Newton step on both and Markstein's correction g + (x - g g) h.
+__device__ __forceinline__ float sqrt_rn_mid(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    float g = x * y, h = 0.5f * y;
+    const float e = fmaf(-g, h, 0.5f);
+    g = fmaf(g, e, g);
+    h = fmaf(h, e, h);
+    return fmaf(fmaf(-g, g, x), h, g);
+}
+
+// sqrt(x) correctly rounded for x = 0 and 2^-80 <= x <= 2^80 (the reciprocal
+// square root's input clamped at 2^-80, so x = 0 gives g = 0 and returns 0);
+// 0 < x < 2^-80 takes sqrtf.
+__device__ __forceinline__ float sqrt_rn_d(float x) {
+    if (__builtin_expect(x < 0x1p-80f && x > 0.0f, 0)) return sqrtf(x);
+    const float y = __builtin_amdgcn_rsqf(fmaxf(x, 0x1p-80f));
+    float g = x * y, h = 0.5f * y;
+    const float e = fmaf(-g, h, 0.5f);
+    g = fmaf(g, e, g);
+    h = fmaf(h, e, h);
+    return fmaf(fmaf(-g, g, x), h, g);
+}
+
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
     const float f = clampf_nan(floorf((p - lo) / H), -1.0f, (float)dim);
     return min(max((int)f, 0), dim - 1);
@@ -412,7 +437,7 @@ __global__ void __launch_bounds__(BS) k2_density(Dev2 P, float dt, DensityArgs A
                 auto cand = [&](const float4 pj) {
                     const float dx = x - pj.x, dy = y - pj.y;
                     const float r2 = fmaf(dy, dy, dx * dx);
-                    if (r2 < P.HSQ) acc += W_cubic(P, sqrt_rn(r2));
+                    if (r2 < P.HSQ) acc += W_cubic(P, sqrt_rn_d(r2));
                 };
                 // kDensityBatch2D candidates loaded before the first is
                 // tested; tested in order
@@ -482,11 +507,18 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                 const float dx = x - qj.x, dy = y - qj.y;
                 const float r2 = fmaf(dy, dy, dx * dx);
                 if (r2 < P.T && r2 > 0.0f) {  // r2 > 0 also excludes j == i
-                    const float r = sqrt_rn(r2);
+                    // r2 < T <= 2^80 (h <= 2^40); below 2^-80 the library square
+                    // root and division
+                    const bool wide = r2 < 0x1p-80f;
+                    float r, inv;
+                    if (__builtin_expect(wide, 0)) {
+                        r = sqrtf(r2);
+                        inv = 1.0f / r;
+                    } else {
+                        r = sqrt_rn_mid(r2);
+                        inv = recip_rn(r);
+                    }
                     const float wg = W_cubic_grad(P, r);
-                    // outside [2^-40, 2^40] the division itself
-                    const bool wide = r2 < 0x1p-80f || r2 > 0x1p80f;
-                    const float inv = __builtin_expect(wide, 0) ? 1.0f / r : recip_rn(r);
                     const float ux = dx * inv, uy = dy * inv;
                     const float s = (ai + bj.x) * wg;
                     apx -= s * ux;
@@ -743,7 +775,7 @@ static Dev2 make_dev2(const cwa2d_params& p) {
 
 static void validate(const cwa2d_params& p) {
     auto fin = [](float v) { return std::isfinite(v); };
-    if (!(p.h > 0.0f) || !fin(p.h)) throw Error(CWA_EINVAL, "h must be positive");
+    if (!(p.h > 0.0f) || !(p.h <= 0x1p40f)) throw Error(CWA_EINVAL, "h must be in (0, 2^40]");
     if (!(p.rest_rho > 0.0f) || !fin(p.rest_rho)) throw Error(CWA_EINVAL, "rest_rho must be positive");
     if (!(p.hi[0] > p.lo[0]) || !(p.hi[1] > p.lo[1]) || !fin(p.lo[0]) || !fin(p.hi[0]) ||
         !fin(p.lo[1]) || !fin(p.hi[1]))

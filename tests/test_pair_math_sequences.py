@@ -1,0 +1,58 @@
+"""The batched 2D pair loops compute 1/r and sqrt(r^2) without the library
+sequences (recip_rn, sqrt_rn_mid, sqrt_rn_d in csrc/cwa_batch2d.hip): a
+hardware estimate refined by Newton-Raphson steps in fused multiply-adds.
+These host checks run the same steps from every start within 3 ulp of the
+answer and demand IEEE 1.0f / b and sqrtf(x) bit for bit, over every float
+of several binades (the steps scale exactly by powers of two in the ranges
+the kernels use them in).  The reciprocal check also shows its fix-up is
+needed: without it the all-ones mantissas fail."""
+import shutil
+import subprocess
+
+import pytest
+
+from pathlib import Path
+
+CPP = Path(__file__).parent / "cpp"
+
+
+def _build(tmp_path_factory, name):
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    exe = tmp_path_factory.mktemp(name) / name
+    # no contraction: the fmaf calls are the only fused operations
+    subprocess.run([gcc, "-O2", "-ffp-contract=off", "-o", str(exe), str(CPP / f"{name}.c"), "-lm"],
+                   check=True)
+    return exe
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    return _build(tmp_path_factory, "recip_rn_check")
+
+
+@pytest.fixture(scope="module")
+def sqrt_checker(tmp_path_factory):
+    return _build(tmp_path_factory, "sqrt_rn_check")
+
+
+@pytest.mark.parametrize("binades", [(-1, 1), (-40, -40), (40, 40)])
+def test_reciprocal_steps_are_correctly_rounded(checker, binades):
+    out = subprocess.run([str(checker), str(binades[0]), str(binades[1]), "3"],
+                         capture_output=True, text=True, timeout=300)
+    checked, bad, bad_unfixed = map(int, out.stdout.split())
+    nb = binades[1] - binades[0] + 1
+    assert checked == nb * (1 << 23) * 7
+    assert bad == 0 and out.returncode == 0
+    # the all-ones mantissa of each binade fails without the fix-up
+    assert bad_unfixed > 0
+
+
+@pytest.mark.parametrize("binades", [(0, 2), (-80, -78), (78, 80)])
+def test_square_root_steps_are_correctly_rounded(sqrt_checker, binades):
+    out = subprocess.run([str(sqrt_checker), str(binades[0]), str(binades[1]), "3"],
+                         capture_output=True, text=True, timeout=300)
+    checked, bad, zero_ok = map(int, out.stdout.split())
+    assert checked == (binades[1] - binades[0]) * (1 << 23) * 7
+    assert bad == 0 and zero_ok == 1 and out.returncode == 0
