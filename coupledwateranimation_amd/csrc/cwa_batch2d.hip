@@ -175,16 +175,14 @@ __device__ __forceinline__ float sqrt_rn(float x) {
 }
 
 // 1 / b correctly rounded (bit-identical to IEEE division) for
-// 2^-40 <= b <= 2^40: the hardware reciprocal and two Newton-Raphson steps
-// y + y (1 - b y), each with two fused multiply-adds.  From a start within
-// 3 ulp these give the correctly rounded reciprocal for every b in that
+// 2^-40 <= b <= 2^40, from a start y within 3 ulp of it: two Newton-Raphson
+// steps y + y (1 - b y), each with two fused multiply-adds.  These give the correctly rounded reciprocal for every b in that
 // range but one kind, a mantissa of all ones, b = (2 - 2^-23) 2^e: there
 // 1 / b lies just above the midpoint 2^-(e+1) (1 + 2^-24), the step lands
 // exactly on it and rounds to the even 2^-(e+1), whose last mantissa bit
 // is the one missing.  The carry out of mantissa + 1 is that bit.  Checked
 // exhaustively over all 81 binades for starts up to 3 ulp either side.
-__device__ __forceinline__ float recip_rn(float b) {
-    float y = __builtin_amdgcn_rcpf(b);
+__device__ __forceinline__ float recip_rn(float b, float y) {
     y = fmaf(fmaf(-b, y, 1.0f), y, y);
     y = fmaf(fmaf(-b, y, 1.0f), y, y);
     const unsigned ub = __float_as_uint(b);
@@ -193,13 +191,16 @@ __device__ __forceinline__ float recip_rn(float b) {
 
 // sqrt(x) correctly rounded for 2^-96 <= x <= 2^96, without compares:
 // the hardware reciprocal square root y, g = x y and h = y / 2, one coupled
-// Newton step on both and Markstein's correction g + (x - g g) h.
-__device__ __forceinline__ float sqrt_rn_mid(float x) {
+// Newton step on both and Markstein's correction g + (x - g g) h.  2h is
+// then within 2 ulp of 1 / sqrt(x) rounded (checked exhaustively), a start
+// for recip_rn that saves the hardware reciprocal.
+__device__ __forceinline__ float sqrt_rn_mid(float x, float& half_rsqrt) {
     const float y = __builtin_amdgcn_rsqf(x);
     float g = x * y, h = 0.5f * y;
     const float e = fmaf(-g, h, 0.5f);
     g = fmaf(g, e, g);
     h = fmaf(h, e, h);
+    half_rsqrt = h;
     return fmaf(fmaf(-g, g, x), h, g);
 }
 
@@ -516,8 +517,9 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                         r = sqrtf(r2);
                         inv = 1.0f / r;
                     } else {
-                        r = sqrt_rn_mid(r2);
-                        inv = recip_rn(r);
+                        float h;
+                        r = sqrt_rn_mid(r2, h);
+                        inv = recip_rn(r, h + h);  // 2h: within 2 ulp of 1 / r
                     }
                     const float wg = W_cubic_grad(P, r);
                     const float ux = dx * inv, uy = dy * inv;
