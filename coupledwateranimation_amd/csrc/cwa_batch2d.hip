@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                 const float dx = x - qj.x, dy = y - qj.y;
                 const float r2 = fmaf(dy, dy, dx * dx);
                 if (r2 < P.T && r2 > 0.0f) {  // r2 > 0 also excludes j == i
-                    // r2 < T <= 2^80 (h <= 2^40); below 2^-80 the library square
+                    // r2 < T <= 2^38 (h <= 2^19); below 2^-80 the library square
                     // root and division
                     const bool wide = r2 < 0x1p-80f;
                     float r, inv;
@@ -527,7 +527,10 @@ __global__ void __launch_bounds__(BS) k2_force(Dev2 P, float dt, ForceArgs A) {
                     apx -= s * ux;
                     apy -= s * uy;
                     const float vdr = (vx - qj.z) * dx + (vy - qj.w) * dy;
-                    const float tv = bj.y * vdr / (r2 + P.c01) * wg;
+                    // dot(vij, rij) / (r^2 + 0.01 H^2) as dot * (1 / ...); the
+                    // denominator lies in [2^-40, 2^40] for 2^-16 <= H <= 2^19
+                    const float den = r2 + P.c01;
+                    const float tv = bj.y * vdr * recip_rn(den, __builtin_amdgcn_rcpf(den)) * wg;
                     avx -= tv * ux;
                     avy -= tv * uy;
                 }
@@ -778,7 +781,8 @@ static Dev2 make_dev2(const cwa2d_params& p) {
 
 static void validate(const cwa2d_params& p) {
     auto fin = [](float v) { return std::isfinite(v); };
-    if (!(p.h > 0.0f) || !(p.h <= 0x1p40f)) throw Error(CWA_EINVAL, "h must be in (0, 2^40]");
+    // the pair loops' square root and reciprocals are exact in this range
+    if (!(p.h >= 0x1p-16f) || !(p.h <= 0x1p19f)) throw Error(CWA_EINVAL, "h must be in [2^-16, 2^19]");
     if (!(p.rest_rho > 0.0f) || !fin(p.rest_rho)) throw Error(CWA_EINVAL, "rest_rho must be positive");
     if (!(p.hi[0] > p.lo[0]) || !(p.hi[1] > p.lo[1]) || !fin(p.lo[0]) || !fin(p.hi[0]) ||
         !fin(p.lo[1]) || !fin(p.hi[1]))

@@ -41,7 +41,7 @@ extern "C" {
 /* Batch-wide physics.  Shader line numbers refer to
  * SphWave2D/SphWaveKoschier2D_grid_cs.glsl unless noted. */
 typedef struct cwa2d_params {
-    float h;               /* support radius H (:120), 0 < H <= 2^40; grid cell = H */
+    float h;               /* support radius H (:120), 2^-16 <= H <= 2^19; cell = H */
     float rest_rho;        /* REST_DENS (:122)                                   */
     float gas_const;       /* GAS_CONST (:126); p = k((rho/rho0)^3 - 1) (:303)   */
     float visc;            /* VISC (:123)                                        */

@@ -24,6 +24,7 @@
 //   pow(rho/rho0, 3)   -> (t*t)*t
 //   length(rij)        -> sqrt(fma(dy, dy, dx*dx)), correctly rounded
 //   rij / r            -> rij * (1/r)
+//   dot(vij, rij) / (r*r + 0.01 HSQ) -> dot * (1/(r2 + 0.01 HSQ))
 //   r*r in viscosity   -> the same fma chain r2
 //   texture(1D, LINEAR, CLAMP) -> fp32 weights, t = u*L - 1/2
 // Additions the reference leaves undefined (recorded as decisions):
@@ -429,7 +430,7 @@ static void step_scene(const K2<R>& c, Scene2<R>& S, R dt) {
                     apx -= s * ux;
                     apy -= s * uy;
                     const R vdr = (vx - qvx[j]) * dx + (vy - qvy[j]) * dy;
-                    const R t = ir[j] * vdr / (r2 + c.c01) * wg;
+                    const R t = ir[j] * vdr * (R(1) / (r2 + c.c01)) * wg;
                     avx -= t * ux;
                     avy -= t * uy;
                 }

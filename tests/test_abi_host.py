@@ -47,9 +47,10 @@ def test_2d_struct_sizes_and_rejections(cwa):
     p = spec.params.copy()
     p.h = 0.0
     bad.append((p, spec.scenes()))
-    p = spec.params.copy()
-    p.h = 2.0 ** 41       # beyond the pair kernels' square root and reciprocal range
-    bad.append((p, spec.scenes()))
+    for h in (2.0 ** 20, 2.0 ** -17):   # outside the pair kernels' exact-arithmetic range
+        p = spec.params.copy()
+        p.h = h
+        bad.append((p, spec.scenes()))
     sc = spec.scenes()
     sc[1].wave_len = 2
     bad.append((spec.params, sc))
