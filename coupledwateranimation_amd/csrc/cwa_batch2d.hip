@@ -66,7 +66,9 @@ constexpr int kNumQueues = 8;   // XCDs of an MI355X
 // 1.42, 1.25, 1.16, 1.13, 1.12, 1.13 ms; force 1, 2, 3, 4, 6, 8 ->
 // 2.63, 2.58, 2.51, 2.49, 2.45, 2.45 ms.  With the shorter columns of H/2
 // cells: density 4 / 8 -> 0.921 / 0.926 ms; force 4, 5, 6, 8 -> 1.885,
-// 1.884, 1.898, 1.980 ms.
+// 1.884, 1.898, 1.980 ms.  Again after the cheaper pair arithmetic (refined
+// hardware square roots and reciprocals): density 3, 4, 6 -> 907, 899,
+// 899 us; force 3, 4, 6 -> 1555, 1532, 1546 us.
 constexpr int kDensityBatch2D = 4;
 constexpr int kForceBatch2D = 4;
 constexpr int kRunShift = 26;   // run record: slot | (len - 1) << 26
