@@ -84,7 +84,7 @@ struct Scene2D {
 // Kernel-side parameters (derived constants in float, as the oracle).
 struct Dev2 {
     float H, HSQ, invH, T;          // T: sqrt_rn(r2) < H  <=>  r2 < T
-    float k2, k2x2, c6k2, m6k2, c01;
+    float k2, k2x2, c6k2, m6k2, c01, ga, gb;  // ga, gb: c6k2 / H, m6k2 / H
     float rho0, k, visc_pre, psi, gx, gy;
     float lox, loy, hix, hiy, walls[4];
     float thr, wref, wpress, wdy, wmask, atten, beta, band, kappa;
@@ -109,9 +109,9 @@ __device__ __forceinline__ float W_cubic(const Dev2& c, float r) {
 __device__ __forceinline__ float W_cubic_grad(const Dev2& c, float r) {
     const float q = r * c.invH;
     if (q >= 1.0f) return 0.0f;
-    if (q <= 0.5f) return c.c6k2 * q * (3.0f * q - 2.0f) * c.invH;
+    if (q <= 0.5f) return c.ga * q * (3.0f * q - 2.0f);
     const float q1 = 1.0f - q;
-    return c.m6k2 * (q1 * q1) * c.invH;
+    return c.gb * (q1 * q1);
 }
 
 __device__ __forceinline__ float smoothstep(float e0, float e1, float x) {
@@ -746,6 +746,8 @@ static Dev2 make_dev2(const cwa2d_params& p) {
     d.k2x2 = d.k2 * 2.0f;
     d.c6k2 = 6.0f * d.k2;
     d.m6k2 = -6.0f * d.k2;
+    d.ga = d.c6k2 * d.invH;
+    d.gb = d.m6k2 * d.invH;
     d.c01 = 0.01f * d.HSQ;
     d.rho0 = p.rest_rho;
     d.k = p.gas_const;

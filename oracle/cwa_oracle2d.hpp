@@ -21,6 +21,8 @@
 // the GPU agree bit for bit.  Where GLSL leaves the arithmetic open (the
 // precision of '/', sqrt, pow, texture filtering) the restatement fixes it:
 //   r/H                -> r * (1/H)              (GLSL allows x * (1/y))
+//   6 k2 q (3q-2) / H, -6 k2 q1^2 / H (W_cubic_grad) -> (6 k2 (1/H)) q (3q-2),
+//                         (-6 k2 (1/H)) q1^2     (constants folded)
 //   pow(rho/rho0, 3)   -> (t*t)*t
 //   length(rij)        -> sqrt(fma(dy, dy, dx*dx)), correctly rounded
 //   rij / r            -> rij * (1/r)
@@ -77,7 +79,7 @@ static inline int grid_dim(float lo, float hi, float h) {
 
 template <class R>
 struct K2 {
-    R H, HSQ, invH, k2, k2x2, c6k2, m6k2, c01;
+    R H, HSQ, invH, k2, k2x2, c6k2, m6k2, c01, ga, gb;  // ga, gb: c6k2 / H, m6k2 / H
     R rho0, k, visc_pre, psi, gx, gy;
     R lox, loy, hix, hiy, walls[4];
     R thr, wref, wpress, wdy, wmask, atten, beta, band, kappa;
@@ -97,6 +99,8 @@ struct K2 {
         k2x2 = k2 * R(2);
         c6k2 = R(6) * k2;
         m6k2 = R(-6) * k2;
+        ga = c6k2 * invH;
+        gb = m6k2 * invH;
         c01 = R(0.01f) * HSQ;
         rho0 = R(p.rest_rho);
         k = R(p.gas_const);
@@ -148,9 +152,9 @@ template <class R>
 static inline R W_cubic_grad(const K2<R>& c, R r) {
     const R q = r * c.invH;
     if (q >= R(1)) return R(0);
-    if (q <= R(0.5)) return c.c6k2 * q * (R(3) * q - R(2)) * c.invH;
+    if (q <= R(0.5)) return c.ga * q * (R(3) * q - R(2));
     const R q1 = R(1) - q;
-    return c.m6k2 * (q1 * q1) * c.invH;
+    return c.gb * (q1 * q1);
 }
 
 // GLSL smoothstep (GLSL 4.50 spec 8.3): t = clamp((x-e0)/(e1-e0), 0, 1).
