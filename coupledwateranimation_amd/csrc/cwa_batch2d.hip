@@ -206,18 +206,20 @@ __device__ __forceinline__ float sqrt_rn_mid(float x, float& half_rsqrt) {
     return fmaf(fmaf(-g, g, x), h, g);
 }
 
-// sqrt(x) correctly rounded for x = 0 and 2^-80 <= x <= 2^80 (the reciprocal
-// square root's input clamped at 2^-80, so x = 0 gives g = 0 and returns 0);
-// 0 < x < 2^-80 takes sqrtf.
+// The density kernel W at r = sqrt(x), x >= 0: for x <= 2^-80 the argument
+// is clamped to 2^-80 (r = 2^-40), which leaves W unchanged: with
+// H >= 2^-16, q = r / H <= 2^-24 there, 6 q^2 (q - 1) stays below half an
+// ulp of 1 and W rounds to k2 for every such r, 0 included.  Above, r is
+// sqrt_rn_mid's correctly rounded square root (x < HSQ <= 2^38).
 __device__ __forceinline__ float sqrt_rn_d(float x) {
-    if (__builtin_expect(x < 0x1p-80f && x > 0.0f, 0)) return sqrtf(x);
     // max on the bit patterns: the same as fmaxf for x >= 0, one instruction
-    const float y = __builtin_amdgcn_rsqf(__uint_as_float(max(__float_as_uint(x), 0x17800000u)));
-    float g = x * y, h = 0.5f * y;
+    const float xc = __uint_as_float(max(__float_as_uint(x), 0x17800000u));
+    const float y = __builtin_amdgcn_rsqf(xc);
+    float g = xc * y, h = 0.5f * y;
     const float e = fmaf(-g, h, 0.5f);
     g = fmaf(g, e, g);
     h = fmaf(h, e, h);
-    return fmaf(fmaf(-g, g, x), h, g);
+    return fmaf(fmaf(-g, g, xc), h, g);
 }
 
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {

@@ -58,3 +58,25 @@ def test_square_root_steps_are_correctly_rounded(sqrt_checker, binades):
     assert bad == 0 and zero_ok == 1 and out.returncode == 0
     # 2h seeds the reciprocal steps, which are checked from starts within 3 ulp
     assert max_seed <= 3
+
+
+@pytest.mark.parametrize("h", [2.0 ** -16, 2.0 ** -16 * 1.7, 0.02, 0.3, 1.0, 2.0 ** 19])
+def test_density_kernel_flat_below_the_square_root_clamp(h):
+    """The batched density pass clamps r^2 at 2^-80 (r = 2^-40) before its
+    square root.  That is exact because W(r) rounds to k2 for every
+    r <= 2^-40 when 2^-16 <= H <= 2^19: the same float operations as the
+    kernel's first piece, k2 (6 q q (q - 1) + 1) with q = r (1/H)."""
+    import numpy as np
+
+    f = np.float32
+    H = f(h)
+    HSQ = H * H
+    invH = f(1) / H
+    k2 = f(40) / (f(7) * f(3.14159265) * HSQ)
+    tiny = np.array([0.0, 1e-45, 1e-40, 2.0 ** -100, 2.0 ** -60, 2.0 ** -40.5,
+                     np.nextafter(f(2.0 ** -40), f(0)), 2.0 ** -40], np.float32)
+    for r in tiny:
+        q = r * invH
+        assert q <= f(0.5)
+        w = k2 * (f(6) * q * q * (q - f(1)) + f(1))
+        assert w == k2, (h, r, w, k2)
