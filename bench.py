@@ -76,6 +76,10 @@ def parse():
                     help="steps of the forward+adjoint measurement (0 disables it)")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="steps of the batched 2D measurement (0 disables it)")
+    ap.add_argument("--batched-tuning", default=None, metavar="JSON",
+                    help="fixed launch shapes of the batched form instead of the autotuner "
+                         "(keys as the tuner reports them), e.g. for counter runs whose "
+                         "collection perturbs the tuner's timings")
     ap.add_argument("--batched-cpu-steps", type=int, default=16,
                     help="oracle steps of the batched CPU baseline, about 10 s on 16 host "
                          "cores (0 disables it)")
@@ -277,10 +281,16 @@ def measure_batched(args) -> dict:
     cell_sub = int((spec.wave_len * spec.wave_substeps).sum())
     b = spec.make()
     t_tune = time.perf_counter()
-    tuning = b.tune(spec.dt, 3)
+    if args.batched_tuning:
+        fixed = {k: v for k, v in json.loads(args.batched_tuning).items() if k != "ms_per_step"}
+        b.set_tuning(**fixed)
+        tuning = b.tuning()
+    else:
+        tuning = b.tune(spec.dt, 3)
     t_tune = time.perf_counter() - t_tune
     print(f"bench: batched {spec.nscenes} scenes, {n} particles, {nc} wave cells; "
-          f"tuned in {t_tune:.2f} s: {tuning}", flush=True)
+          f"{'fixed' if args.batched_tuning else 'tuned'} in {t_tune:.2f} s: {tuning}",
+          flush=True)
     b.step(spec.dt, 5, graph=True)  # warm-up (graph instantiated, caches warm)
     times = []
     for _ in range(max(1, min(args.repeats, 3))):
