@@ -178,11 +178,12 @@ __device__ __forceinline__ float sqrt_rn(float x) {
 
 // 1 / b correctly rounded (bit-identical to IEEE division) for
 // 2^-40 <= b <= 2^40, from a start y within 3 ulp of it: two Newton-Raphson
-// steps y + y (1 - b y), each with two fused multiply-adds.  These give the correctly rounded reciprocal for every b in that
-// range but one kind, a mantissa of all ones, b = (2 - 2^-23) 2^e: there
-// 1 / b lies just above the midpoint 2^-(e+1) (1 + 2^-24), the step lands
-// exactly on it and rounds to the even 2^-(e+1), whose last mantissa bit
-// is the one missing.  The carry out of mantissa + 1 is that bit.  Checked
+// steps y + y (1 - b y), each with two fused multiply-adds.  These give the
+// correctly rounded reciprocal for every b in that range but one kind, a
+// mantissa of all ones, b = (2 - 2^-23) 2^e: there 1 / b lies just above
+// the midpoint 2^-(e+1) (1 + 2^-24), the step lands exactly on it and
+// rounds to the even 2^-(e+1), whose last mantissa bit is the one missing.
+// The carry out of mantissa + 1 is that bit.  Checked
 // exhaustively over all 81 binades for starts up to 3 ulp either side.
 __device__ __forceinline__ float recip_rn(float b, float y) {
     y = fmaf(fmaf(-b, y, 1.0f), y, y);
@@ -206,20 +207,14 @@ __device__ __forceinline__ float sqrt_rn_mid(float x, float& half_rsqrt) {
     return fmaf(fmaf(-g, g, x), h, g);
 }
 
-// The density kernel W at r = sqrt(x), x >= 0: for x <= 2^-80 the argument
-// is clamped to 2^-80 (r = 2^-40), which leaves W unchanged: with
-// H >= 2^-16, q = r / H <= 2^-24 there, 6 q^2 (q - 1) stays below half an
-// ulp of 1 and W rounds to k2 for every such r, 0 included.  Above, r is
-// sqrt_rn_mid's correctly rounded square root (x < HSQ <= 2^38).
-__device__ __forceinline__ float sqrt_rn_d(float x) {
+// r = sqrt(max(x, 2^-80)) correctly rounded, for 0 <= x < 2^38: the density
+// pass's r.  The clamp (r = 2^-40 for x <= 2^-80) leaves W(r) unchanged:
+// with H >= 2^-16, q = r / H <= 2^-24 there, 6 q^2 (q - 1) stays below half
+// an ulp of 1 and W rounds to k2 for every such r, 0 included.
+__device__ __forceinline__ float sqrt_rn_clamped(float x) {
     // max on the bit patterns: the same as fmaxf for x >= 0, one instruction
-    const float xc = __uint_as_float(max(__float_as_uint(x), 0x17800000u));
-    const float y = __builtin_amdgcn_rsqf(xc);
-    float g = xc * y, h = 0.5f * y;
-    const float e = fmaf(-g, h, 0.5f);
-    g = fmaf(g, e, g);
-    h = fmaf(h, e, h);
-    return fmaf(fmaf(-g, g, xc), h, g);
+    float h;
+    return sqrt_rn_mid(__uint_as_float(max(__float_as_uint(x), 0x17800000u)), h);
 }
 
 __device__ __forceinline__ int cell2(float p, float lo, float H, int dim) {
@@ -443,7 +438,7 @@ __global__ void __launch_bounds__(BS) k2_density(Dev2 P, float dt, DensityArgs A
                 auto cand = [&](const float4 pj) {
                     const float dx = x - pj.x, dy = y - pj.y;
                     const float r2 = fmaf(dy, dy, dx * dx);
-                    if (r2 < P.HSQ) acc += W_cubic(P, sqrt_rn_d(r2));
+                    if (r2 < P.HSQ) acc += W_cubic(P, sqrt_rn_clamped(r2));
                 };
                 // kDensityBatch2D candidates loaded before the first is
                 // tested; tested in order

@@ -1,5 +1,5 @@
 /* Host check of the square root of the batched 2D pair loops (sqrt_rn_mid /
- * sqrt_rn_d in coupledwateranimation_amd/csrc/cwa_batch2d.hip): from a
+ * sqrt_rn_clamped in coupledwateranimation_amd/csrc/cwa_batch2d.hip): from a
  * reciprocal square root y within K ulp of RN(1/sqrt x) (the hardware
  * estimate's place), g = x y, h = y / 2, one coupled Newton step and the
  * correction g + (x - g g) h give IEEE sqrtf(x) exactly.  With the estimate

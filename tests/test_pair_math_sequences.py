@@ -1,5 +1,5 @@
 """The batched 2D pair loops compute 1/r and sqrt(r^2) without the library
-sequences (recip_rn, sqrt_rn_mid, sqrt_rn_d in csrc/cwa_batch2d.hip): a
+sequences (recip_rn, sqrt_rn_mid, sqrt_rn_clamped in csrc/cwa_batch2d.hip): a
 hardware estimate refined by Newton-Raphson steps in fused multiply-adds.
 These host checks run the same steps from every start within 3 ulp of the
 answer and demand IEEE 1.0f / b and sqrtf(x) bit for bit, over every float
