@@ -106,9 +106,13 @@ __device__ __forceinline__ float W_cubic(const Dev2& c, float r) {
     return c.k2x2 * q1 * q1 * q1;
 }
 
+// Called with r < H only (pairs with r2 < T, walls with d < H).  Then
+// q = r (1/H) <= 1: the largest float below H times the rounded 1/H is at
+// most 1 before rounding.  q >= 1 would be q == 1 exactly, where the second
+// piece gives -0 and the shader's early return +0; the sums that take W'
+// never hold -0, so both add the same.  Hence no q >= 1 test here.
 __device__ __forceinline__ float W_cubic_grad(const Dev2& c, float r) {
     const float q = r * c.invH;
-    if (q >= 1.0f) return 0.0f;
     if (q <= 0.5f) return c.ga * q * (3.0f * q - 2.0f);
     const float q1 = 1.0f - q;
     return c.gb * (q1 * q1);

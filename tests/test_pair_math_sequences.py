@@ -80,3 +80,20 @@ def test_density_kernel_flat_below_the_square_root_clamp(h):
         assert q <= f(0.5)
         w = k2 * (f(6) * q * q * (q - f(1)) + f(1))
         assert w == k2, (h, r, w, k2)
+
+
+def test_kernel_gradient_argument_stays_at_most_one():
+    """The batched force pass evaluates W'(r) only for r < H and drops the
+    q >= 1 early return: q = r * (1/H) with both rounded to float is then at
+    most 1.  Checked for the largest float below H (q grows with r) over
+    every mantissa of one binade and random H across [2^-16, 2^19]."""
+    import numpy as np
+
+    m = np.arange(1 << 23, dtype=np.uint32)
+    H = (np.uint32(127 << 23) | m).view(np.float32)           # [1, 2)
+    rng = np.random.default_rng(0)
+    H = np.concatenate([H, rng.uniform(2.0 ** -16, 2.0 ** 19, 1 << 20).astype(np.float32),
+                        np.float32([2.0 ** -16, 2.0 ** 19])])
+    r = np.nextafter(H, np.float32(0))
+    q = r * (np.float32(1) / H)
+    assert q.dtype == np.float32 and float(q.max()) <= 1.0
