@@ -264,6 +264,26 @@ def measure_fit(args) -> dict:
     }
 
 
+TUNING_KEYS = ("block_density", "block_force", "grid_density", "grid_force", "xcd_queues",
+               "wave_block")
+
+
+def fixed_tuning(text: str) -> dict:
+    """--batched-tuning: a JSON object of launch shapes with the autotuner's
+    keys (its ms_per_step, if copied along, is dropped); unknown keys or
+    non-integer values are an error."""
+    d = json.loads(text)
+    if not isinstance(d, dict):
+        raise ValueError("--batched-tuning takes a JSON object")
+    d.pop("ms_per_step", None)
+    bad = [k for k in d if k not in TUNING_KEYS]
+    if bad:
+        raise ValueError(f"--batched-tuning: unknown keys {bad}; known: {list(TUNING_KEYS)}")
+    if not all(isinstance(v, int) for v in d.values()):
+        raise ValueError("--batched-tuning: values must be integers")
+    return d
+
+
 def measure_batched(args) -> dict:
     """The batched 2D form: 256 independent 2D SPH + 1D-wave scenes (N_s
     log-uniform in [1k, 64k], columns log-uniform in [256, 4096] cells),
@@ -282,8 +302,7 @@ def measure_batched(args) -> dict:
     b = spec.make()
     t_tune = time.perf_counter()
     if args.batched_tuning:
-        fixed = {k: v for k, v in json.loads(args.batched_tuning).items() if k != "ms_per_step"}
-        b.set_tuning(**fixed)
+        b.set_tuning(**fixed_tuning(args.batched_tuning))
         tuning = b.tuning()
     else:
         tuning = b.tune(spec.dt, 3)

@@ -87,3 +87,15 @@ def test_search_stats_against_brute_force(oracle):
     one = np.array([[0.5, 0.5, 0.5, 0.0]], np.float32)
     s1 = b.search_stats(spec.params, one, np.array([1]))
     assert s1["candidates_per_particle"] == 1.0
+
+
+def test_fixed_batched_tuning_parsing():
+    bench = _bench()
+    tuned = {"block_density": 512, "block_force": 512, "grid_density": 1024,
+             "grid_force": 1024, "xcd_queues": 1, "wave_block": 1024, "ms_per_step": 2.47}
+    d = bench.fixed_tuning(json.dumps(tuned))
+    assert "ms_per_step" not in d and d["grid_force"] == 1024 and len(d) == 6
+    assert bench.fixed_tuning('{"xcd_queues": 0}') == {"xcd_queues": 0}
+    for bad in ('[512]', '{"block": 512}', '{"grid_force": 1.5}'):
+        with pytest.raises(ValueError):
+            bench.fixed_tuning(bad)
