@@ -98,9 +98,12 @@ struct Dev2 {
 // ---------------------------------------------------------------------------
 // device helpers (each mirrors the oracle function of the same name)
 // ---------------------------------------------------------------------------
+// Called with r = sqrt of an r2 < HSQ (pairs) or r = d < H (walls); both
+// give q = r (1/H) <= 1 (the largest such r times the rounded 1/H rounds to
+// at most 1), and at q == 1 the second piece is +0, as the shader's early
+// return.  Hence no q >= 1 test here.
 __device__ __forceinline__ float W_cubic(const Dev2& c, float r) {
     const float q = r * c.invH;
-    if (q >= 1.0f) return 0.0f;
     if (q <= 0.5f) return c.k2 * (6.0f * q * q * (q - 1.0f) + 1.0f);
     const float q1 = 1.0f - q;
     return c.k2x2 * q1 * q1 * q1;

@@ -97,3 +97,21 @@ def test_kernel_gradient_argument_stays_at_most_one():
     r = np.nextafter(H, np.float32(0))
     q = r * (np.float32(1) / H)
     assert q.dtype == np.float32 and float(q.max()) <= 1.0
+
+
+def test_kernel_argument_of_the_density_test_stays_at_most_one():
+    """The batched density pass evaluates W(sqrt(r2)) only for r2 < HSQ (the
+    shader's test) and drops W's q >= 1 early return: the largest float
+    below HSQ = H*H, its correctly rounded square root times the rounded
+    1/H, is at most 1 (q grows with r2), and at q == 1 the second piece
+    k2x2 (1-q)^3 is +0 like the early return.  Every mantissa of one
+    binade of H (q does not change when H scales by a power of two)."""
+    import numpy as np
+
+    m = np.arange(1 << 23, dtype=np.uint32)
+    H = (np.uint32(127 << 23) | m).view(np.float32)
+    r2 = np.nextafter(H * H, np.float32(0))
+    q = np.sqrt(r2) * (np.float32(1) / H)
+    assert q.dtype == np.float32 and float(q.max()) <= 1.0
+    q1 = np.float32(1) - np.float32(1)
+    assert (np.float32(3.0) * q1 * q1 * q1) == 0 and not np.signbit(np.float32(3.0) * q1 * q1 * q1)
