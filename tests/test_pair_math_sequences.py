@@ -5,7 +5,10 @@ These host checks run the same steps from every start within 3 ulp of the
 answer and demand IEEE 1.0f / b and sqrtf(x) bit for bit, over every float
 of several binades (the steps scale exactly by powers of two in the ranges
 the kernels use them in).  The reciprocal check also shows its fix-up is
-needed: without it the all-ones mantissas fail."""
+needed: without it the all-ones mantissas fail.  The last checks pin the
+float facts that let the pair loops drop branches of the shader's kernel
+functions without changing a bit: W is flat below the square root's clamp,
+and q = r (1/H) stays at most 1 where the q >= 1 early returns were."""
 import shutil
 import subprocess
 
